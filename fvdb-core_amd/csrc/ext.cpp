@@ -1,0 +1,324 @@
+// PyTorch-ROCm extension over the fvdbk C ABI.
+//
+// This is the tensor-level layer that the reference implements as the templated
+// ops::dispatch*<DeviceTag> functions on torch::Tensor (e.g.
+// src/fvdb/detail/ops/convolution/pack_info/ConvolutionKernelMap.h:16-21 and the
+// backend/*.h headers): it validates shapes and dtypes, allocates outputs with the
+// caching allocator, takes the current HIP stream, and calls the C ABI.  Status codes
+// become the reference's error classes: FVDBK_EINVAL -> ValueError
+// (TORCH_CHECK_VALUE), FVDBK_ETYPE -> TypeError (TORCH_CHECK_TYPE), others ->
+// RuntimeError.  Host synchronisations happen only where a size must be known on the
+// host (grid build, neighbour-map size), as in the reference.
+#include <torch/extension.h>
+
+#include <c10/hip/HIPGuard.h>
+#include <c10/hip/HIPStream.h>
+
+#include <string>
+#include <vector>
+
+#include "fvdbk.h"
+
+namespace {
+
+void check_rc(int rc, const char* what) {
+    if (rc == FVDBK_OK) return;
+    const std::string msg = std::string(what) + ": " + fvdbk_last_error();
+    if (rc == FVDBK_EINVAL) TORCH_CHECK_VALUE(false, msg);
+    if (rc == FVDBK_ETYPE) TORCH_CHECK_TYPE(false, msg);
+    TORCH_CHECK(false, msg);
+}
+
+hipStream_t stream_of(const at::Tensor& t) { return c10::hip::getCurrentHIPStream(t.device().index()).stream(); }
+
+void require_hip(const at::Tensor& t, const char* name) {
+    TORCH_CHECK_VALUE(t.is_cuda(), name, " must be on a HIP device (fvdb-core_amd has no CPU path)");
+}
+
+fvdbk_dtype to_dtype(at::ScalarType s) {
+    switch (s) {
+    case at::kFloat: return FVDBK_F32;
+    case at::kDouble: return FVDBK_F64;
+    case at::kHalf: return FVDBK_F16;
+    case at::kBFloat16: return FVDBK_BF16;
+    default: TORCH_CHECK_TYPE(false, "unsupported floating point type ", s);
+    }
+    return FVDBK_F32;
+}
+
+// Grid tensors as held by the Python GridBatch: ijk [N,3] i32, leaf_keys [L] i64,
+// leaf_masks [L,8] i64, word_base [L,8] i32, hash [cap,2] i64.
+fvdbk_grid make_grid(const at::Tensor& ijk, const at::Tensor& keys, const at::Tensor& masks, const at::Tensor& wbase,
+                     const at::Tensor& hash, int64_t n_grids) {
+    fvdbk_grid g{};
+    g.n_voxels = ijk.size(0);
+    g.n_leaves = keys.size(0);
+    g.hash_cap = hash.size(0);
+    g.n_grids = (int32_t)n_grids;
+    g.ijk = ijk.data_ptr<int32_t>();
+    g.leaf_keys = reinterpret_cast<const uint64_t*>(keys.data_ptr<int64_t>());
+    g.leaf_masks = reinterpret_cast<const uint64_t*>(masks.data_ptr<int64_t>());
+    g.leaf_word_base = wbase.data_ptr<int32_t>();
+    g.hash = reinterpret_cast<const fvdbk_hslot*>(hash.data_ptr<int64_t>());
+    return g;
+}
+
+std::vector<int32_t> vec3(const std::vector<int64_t>& v, const char* name) {
+    TORCH_CHECK_VALUE(v.size() == 3, name, " must have 3 entries");
+    return {(int32_t)v[0], (int32_t)v[1], (int32_t)v[2]};
+}
+
+int64_t next_pow2(int64_t x) {
+    int64_t p = 16;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+}  // namespace
+
+// Build a grid batch index from voxel coordinates (duplicates merged, entries with a
+// negative grid index skipped).  Returns [ijk, jidx, grid_counts, leaf_keys,
+// leaf_masks, word_base, hash].
+std::vector<at::Tensor> build_grid(at::Tensor ijk, at::Tensor jidx, int64_t n_grids) {
+    require_hip(ijk, "ijk");
+    TORCH_CHECK_VALUE(ijk.dim() == 2 && ijk.size(1) == 3, "ijk must have shape [N, 3]");
+    TORCH_CHECK_TYPE(ijk.scalar_type() == at::kInt, "ijk must be int32");
+    TORCH_CHECK_VALUE(n_grids >= 0 && n_grids <= 1024, "a grid batch holds at most 1024 grids");
+    ijk = ijk.contiguous();
+    const bool has_jidx = jidx.numel() > 0;
+    if (has_jidx) {
+        TORCH_CHECK_VALUE(jidx.numel() == ijk.size(0), "jidx must have one entry per voxel");
+        jidx = jidx.to(at::kInt).contiguous();
+    }
+    c10::hip::HIPGuard guard(ijk.device().index());
+    hipStream_t s = stream_of(ijk);
+    const int64_t n = ijk.size(0);
+    auto o32 = ijk.options().dtype(at::kInt);
+    auto o64 = ijk.options().dtype(at::kLong);
+
+    at::Tensor keys = at::empty({n}, o64);
+    at::Tensor bad = at::zeros({1}, o32);
+    check_rc(fvdbk_grid_leaf_keys(ijk.data_ptr<int32_t>(), has_jidx ? jidx.data_ptr<int32_t>() : nullptr, n,
+                                  reinterpret_cast<uint64_t*>(keys.data_ptr<int64_t>()), bad.data_ptr<int32_t>(), s),
+             "grid_leaf_keys");
+    const int nbad = bad.item<int32_t>();
+    TORCH_CHECK_VALUE(nbad == 0, nbad, " voxel coordinates lie outside [-2^19, 2^19) or have a grid index >= 1024");
+
+    at::Tensor ukeys = std::get<0>(at::_unique(keys, /*sorted=*/true, /*return_inverse=*/false));
+    int64_t L = ukeys.size(0);
+    if (L > 0 && ukeys[L - 1].item<int64_t>() == INT64_MAX) {
+        ukeys = ukeys.slice(0, 0, L - 1);
+        L -= 1;
+    }
+    ukeys = ukeys.contiguous();
+    const int64_t cap = next_pow2(2 * L + 2);
+    at::Tensor hash = at::empty({cap, 2}, o64);
+    check_rc(fvdbk_grid_hash_build(reinterpret_cast<const uint64_t*>(ukeys.data_ptr<int64_t>()), L,
+                                   reinterpret_cast<fvdbk_hslot*>(hash.data_ptr<int64_t>()), cap, s),
+             "grid_hash_build");
+    at::Tensor masks = at::empty({L, 8}, o64);
+    check_rc(fvdbk_grid_leaf_masks(ijk.data_ptr<int32_t>(), has_jidx ? jidx.data_ptr<int32_t>() : nullptr, n,
+                                   reinterpret_cast<const fvdbk_hslot*>(hash.data_ptr<int64_t>()), cap,
+                                   reinterpret_cast<uint64_t*>(masks.data_ptr<int64_t>()), L, s),
+             "grid_leaf_masks");
+    at::Tensor counts = at::empty({L}, o32);
+    check_rc(fvdbk_grid_leaf_counts(reinterpret_cast<const uint64_t*>(masks.data_ptr<int64_t>()), L,
+                                    counts.data_ptr<int32_t>(), s),
+             "grid_leaf_counts");
+    at::Tensor incl = at::cumsum(counts, 0, at::kLong);
+    const int64_t nv = L > 0 ? incl[L - 1].item<int64_t>() : 0;
+    at::Tensor start = (incl - counts.to(at::kLong)).contiguous();
+    at::Tensor wbase = at::empty({L, 8}, o32);
+    at::Tensor ijk_out = at::empty({nv, 3}, o32);
+    at::Tensor jidx_out = at::empty({nv}, o32);
+    at::Tensor gcounts = at::empty({n_grids}, o32);
+    check_rc(fvdbk_grid_finalize(reinterpret_cast<const uint64_t*>(ukeys.data_ptr<int64_t>()),
+                                 reinterpret_cast<const uint64_t*>(masks.data_ptr<int64_t>()), start.data_ptr<int64_t>(),
+                                 L, wbase.data_ptr<int32_t>(), ijk_out.data_ptr<int32_t>(), jidx_out.data_ptr<int32_t>(),
+                                 gcounts.data_ptr<int32_t>(), (int32_t)n_grids, s),
+             "grid_finalize");
+    return {ijk_out, jidx_out, gcounts, ukeys, masks, wbase, hash};
+}
+
+at::Tensor ijk_to_index(at::Tensor g_ijk, at::Tensor g_keys, at::Tensor g_masks, at::Tensor g_wbase, at::Tensor g_hash,
+                        int64_t n_grids, at::Tensor ijk, at::Tensor jidx) {
+    require_hip(ijk, "ijk");
+    TORCH_CHECK_VALUE(ijk.dim() == 2 && ijk.size(1) == 3, "ijk must have shape [N, 3]");
+    ijk = ijk.to(at::kInt).contiguous();
+    const bool has_jidx = jidx.numel() > 0;
+    if (has_jidx) jidx = jidx.to(at::kInt).contiguous();
+    c10::hip::HIPGuard guard(ijk.device().index());
+    fvdbk_grid g = make_grid(g_ijk, g_keys, g_masks, g_wbase, g_hash, n_grids);
+    at::Tensor out = at::empty({ijk.size(0)}, ijk.options().dtype(at::kInt));
+    check_rc(fvdbk_grid_ijk_to_index(&g, ijk.data_ptr<int32_t>(), has_jidx ? jidx.data_ptr<int32_t>() : nullptr,
+                                     ijk.size(0), out.data_ptr<int32_t>(), stream_of(ijk)),
+             "ijk_to_index");
+    return out;
+}
+
+at::Tensor points_to_ijk(at::Tensor points, at::Tensor jidx, at::Tensor transforms) {
+    require_hip(points, "points");
+    TORCH_CHECK_VALUE(points.dim() == 2 && points.size(1) == 3, "points must have shape [N, 3]");
+    TORCH_CHECK_TYPE(points.scalar_type() == at::kFloat || points.scalar_type() == at::kDouble,
+                     "points must be float32 or float64");
+    points = points.contiguous();
+    const bool has_jidx = jidx.numel() > 0;
+    if (has_jidx) jidx = jidx.to(at::kInt).contiguous();
+    transforms = transforms.to(points.device(), at::kDouble).contiguous();
+    c10::hip::HIPGuard guard(points.device().index());
+    at::Tensor out = at::empty({points.size(0), 3}, points.options().dtype(at::kInt));
+    check_rc(fvdbk_points_to_ijk(to_dtype(points.scalar_type()), points.data_ptr(),
+                                 has_jidx ? jidx.data_ptr<int32_t>() : nullptr, points.size(0),
+                                 transforms.data_ptr<double>(), out.data_ptr<int32_t>(), stream_of(points)),
+             "points_to_ijk");
+    return out;
+}
+
+std::vector<at::Tensor> conv_grid_candidates(at::Tensor ijk, at::Tensor jidx, std::vector<int64_t> ks,
+                                             std::vector<int64_t> stride) {
+    require_hip(ijk, "ijk");
+    auto k3 = vec3(ks, "kernel_size");
+    auto s3 = vec3(stride, "stride");
+    ijk = ijk.contiguous();
+    const bool has_jidx = jidx.numel() > 0;
+    if (has_jidx) jidx = jidx.to(at::kInt).contiguous();
+    c10::hip::HIPGuard guard(ijk.device().index());
+    for (int d = 0; d < 3; ++d) {
+        TORCH_CHECK_VALUE(k3[d] > 0, "kernel_size must be strictly positive.");
+        TORCH_CHECK_VALUE(s3[d] > 0, "stride must be strictly positive.");
+    }
+    const int64_t per = fvdbk_conv_grid_candidates_per_voxel(k3.data(), s3.data());
+    const int64_t n = ijk.size(0);
+    at::Tensor cijk = at::empty({n * per, 3}, ijk.options());
+    at::Tensor cj = at::empty({n * per}, ijk.options());
+    check_rc(fvdbk_conv_grid_candidates(ijk.data_ptr<int32_t>(), has_jidx ? jidx.data_ptr<int32_t>() : nullptr, n,
+                                        k3.data(), s3.data(), cijk.data_ptr<int32_t>(), cj.data_ptr<int32_t>(),
+                                        stream_of(ijk)),
+             "conv_grid_candidates");
+    return {cijk, cj};
+}
+
+at::Tensor kernel_map(std::vector<at::Tensor> rows, std::vector<at::Tensor> probe, int64_t n_grids,
+                      std::vector<int64_t> ks, std::vector<int64_t> stride, bool inverse, int64_t n_rows_padded) {
+    TORCH_CHECK_VALUE(rows.size() == 5 && probe.size() == 5, "grid tensors: [ijk, keys, masks, word_base, hash]");
+    require_hip(rows[0], "grid");
+    auto k3 = vec3(ks, "kernel_size");
+    auto s3 = vec3(stride, "stride");
+    c10::hip::HIPGuard guard(rows[0].device().index());
+    fvdbk_grid gr = make_grid(rows[0], rows[1], rows[2], rows[3], rows[4], n_grids);
+    fvdbk_grid gp = make_grid(probe[0], probe[1], probe[2], probe[3], probe[4], n_grids);
+    const int64_t K = (int64_t)k3[0] * k3[1] * k3[2];
+    at::Tensor map = at::empty({n_rows_padded, K}, rows[0].options().dtype(at::kInt));
+    check_rc(fvdbk_kmap_build(&gr, &gp, k3.data(), s3.data(), inverse ? 1 : 0, map.data_ptr<int32_t>(),
+                              n_rows_padded, stream_of(rows[0])),
+             "kmap_build");
+    return map;
+}
+
+// (pairs [P,2] (source index, target row) ordered by (k, row), nbsizes [K]) from a map.
+std::vector<at::Tensor> neighbor_map(at::Tensor map, int64_t n_rows) {
+    require_hip(map, "map");
+    TORCH_CHECK_VALUE(map.dim() == 2 && map.scalar_type() == at::kInt, "map must be int32 [rows, K]");
+    TORCH_CHECK_VALUE(map.size(0) >= n_rows, "map has fewer rows than requested");
+    map = map.contiguous();
+    c10::hip::HIPGuard guard(map.device().index());
+    hipStream_t s = stream_of(map);
+    const int64_t K = map.size(1);
+    const int64_t nblk = (n_rows + FVDBK_NBMAP_ROWS - 1) / FVDBK_NBMAP_ROWS;
+    auto o32 = map.options();
+    if (nblk == 0) return {at::empty({0, 2}, o32), at::zeros({K}, o32)};
+    at::Tensor counts = at::empty({nblk, K}, o32);
+    check_rc(fvdbk_nbmap_block_counts(map.data_ptr<int32_t>(), n_rows, (int32_t)K, counts.data_ptr<int32_t>(), s),
+             "nbmap_block_counts");
+    at::Tensor nbsizes = counts.sum(0, false, at::kInt);
+    // exclusive scan in (k, block) order, laid back out as [block][k]
+    at::Tensor flat = counts.t().contiguous().view({-1}).to(at::kLong);
+    at::Tensor incl = at::cumsum(flat, 0);
+    const int64_t P = incl[incl.size(0) - 1].item<int64_t>();
+    at::Tensor offs = (incl - flat).view({K, nblk}).t().contiguous();
+    at::Tensor pairs = at::empty({P, 2}, o32);
+    check_rc(fvdbk_nbmap_emit(map.data_ptr<int32_t>(), n_rows, (int32_t)K, offs.data_ptr<int64_t>(),
+                              pairs.data_ptr<int32_t>(), s),
+             "nbmap_emit");
+    return {pairs, nbsizes};
+}
+
+// out[r][m] = sum_k sum_c in[map[r][k]][c] * a[k][m][c]
+at::Tensor implicit_gemm(at::Tensor in, at::Tensor a, at::Tensor map, int64_t n_rows) {
+    require_hip(in, "features");
+    TORCH_CHECK_VALUE(in.dim() == 2, "features must have shape [N, C]");
+    TORCH_CHECK_VALUE(a.dim() == 3, "weights must have shape [K, M, C]");
+    TORCH_CHECK_TYPE(in.is_floating_point(), "features must have a floating point type");
+    TORCH_CHECK_TYPE(a.scalar_type() == in.scalar_type(), "features and weights must have the same dtype (",
+                     in.scalar_type(), " vs ", a.scalar_type(), ")");
+    TORCH_CHECK_VALUE(a.size(2) == in.size(1), "weights expect ", a.size(2), " input channels, features have ",
+                      in.size(1));
+    TORCH_CHECK_VALUE(map.dim() == 2 && map.size(1) == a.size(0) && map.size(0) >= n_rows,
+                      "kernel map must have shape [>= rows, K]");
+    TORCH_CHECK_VALUE(map.scalar_type() == at::kInt, "kernel map must be int32");
+    in = in.contiguous();
+    a = a.contiguous();
+    map = map.contiguous();
+    c10::hip::HIPGuard guard(in.device().index());
+    at::Tensor out = at::empty({n_rows, a.size(1)}, in.options());
+    check_rc(fvdbk_conv_implicit_gemm(to_dtype(in.scalar_type()), in.data_ptr(), in.size(0), (int32_t)in.size(1),
+                                      a.data_ptr(), map.data_ptr<int32_t>(), n_rows, (int32_t)a.size(0),
+                                      (int32_t)a.size(1), out.data_ptr(), 0u, stream_of(in)),
+             "conv_implicit_gemm");
+    return out;
+}
+
+// gw [K, Cin, Cout] (fp32, fp64 for fp64 inputs) from the neighbour-map pairs.
+// nbsizes_cpu: int64 CPU tensor [K] (pair counts per offset, known on the host).
+at::Tensor wgrad(at::Tensor in, at::Tensor gout, at::Tensor pairs, at::Tensor nbsizes_cpu, bool transposed,
+                 int64_t chunk) {
+    require_hip(in, "features");
+    TORCH_CHECK_TYPE(in.scalar_type() == gout.scalar_type(), "features and gradients must have the same dtype");
+    TORCH_CHECK_VALUE(in.dim() == 2 && gout.dim() == 2, "features and gradients must be 2-D");
+    TORCH_CHECK_VALUE(pairs.dim() == 2 && pairs.size(1) == 2 && pairs.scalar_type() == at::kInt,
+                      "pairs must be int32 [P, 2]");
+    TORCH_CHECK_VALUE(!nbsizes_cpu.is_cuda(), "nbsizes must be a host tensor");
+    in = in.contiguous();
+    gout = gout.contiguous();
+    pairs = pairs.contiguous();
+    c10::hip::HIPGuard guard(in.device().index());
+    at::Tensor sizes = nbsizes_cpu.to(at::kLong).contiguous();
+    const int64_t K = sizes.size(0);
+    const int64_t* sz = sizes.data_ptr<int64_t>();
+    std::vector<int64_t> koff(K + 1, 0);
+    std::vector<int32_t> cstart(K + 1, 0);
+    for (int64_t k = 0; k < K; ++k) {
+        koff[k + 1] = koff[k] + sz[k];
+        cstart[k + 1] = cstart[k] + (int32_t)((sz[k] + chunk - 1) / chunk);
+    }
+    TORCH_CHECK_VALUE(koff[K] == pairs.size(0), "nbsizes do not add up to the number of pairs");
+    const int64_t total = cstart[K];
+    const fvdbk_dtype dt = to_dtype(in.scalar_type());
+    auto dev = in.options();
+    at::Tensor koff_t = at::from_blob(koff.data(), {K + 1}, at::kLong).to(in.device());
+    at::Tensor cs_t = at::from_blob(cstart.data(), {K + 1}, at::kInt).to(in.device());
+    const size_t ws_bytes = fvdbk_conv_wgrad_workspace_size(dt, total, (int32_t)in.size(1), (int32_t)gout.size(1));
+    at::Tensor ws = at::empty({(int64_t)((ws_bytes + 7) / 8)}, dev.dtype(at::kLong));
+    at::Tensor gw = at::empty({K, in.size(1), gout.size(1)}, dev.dtype(dt == FVDBK_F64 ? at::kDouble : at::kFloat));
+    check_rc(fvdbk_conv_wgrad(dt, in.data_ptr(), (int32_t)in.size(1), gout.data_ptr(), (int32_t)gout.size(1),
+                              pairs.data_ptr<int32_t>(), koff_t.data_ptr<int64_t>(), cs_t.data_ptr<int32_t>(),
+                              (int32_t)K, (int32_t)chunk, total, transposed ? 1 : 0, ws.data_ptr(), ws_bytes,
+                              gw.data_ptr(), stream_of(in)),
+             "conv_wgrad");
+    return gw;
+}
+
+std::string version() { return fvdbk_version(); }
+
+PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
+    m.doc() = "fvdb-core_amd tensor layer over the fvdbk C ABI";
+    m.def("version", &version);
+    m.def("build_grid", &build_grid);
+    m.def("ijk_to_index", &ijk_to_index);
+    m.def("points_to_ijk", &points_to_ijk);
+    m.def("conv_grid_candidates", &conv_grid_candidates);
+    m.def("kernel_map", &kernel_map);
+    m.def("neighbor_map", &neighbor_map);
+    m.def("implicit_gemm", &implicit_gemm);
+    m.def("wgrad", &wgrad);
+}

@@ -1,0 +1,102 @@
+"""Autograd for the sparse convolution: forward, dgrad and wgrad on gfx950 kernels.
+
+Replaces ``src/fvdb/detail/autograd/SparseConvolutionKernelMap.cpp:19-191`` and
+``SparseConvolutionImplicitGEMM.cpp:17-226``.  Weight layouts follow the reference:
+a forward weight ``[O, I, k0, k1, k2]`` is flattened with ``permute(4,3,2,1,0)`` to
+``[K, I, O]`` (``SparseConvolutionKernelMap.cpp:75``), a transposed weight
+``[I, O, k0, k1, k2]`` with ``permute(4,3,2,0,1)`` (``:92``), and the gradient is
+mapped back with the inverse permutations (``:170-187``).
+
+* forward: implicit GEMM over the kernel map (rows = output voxels);
+* dgrad:   the same kernel over the opposite map with the un-transposed weights;
+* wgrad:   split-K MFMA reduction over the neighbour-map pairs with an fp32
+  Neumaier-compensated sum of the chunk partials.
+"""
+from __future__ import annotations
+
+import torch
+
+from ._lib import ext, require_device
+
+
+def weight_to_kio(weight: torch.Tensor, transposed: bool) -> torch.Tensor:
+    w = weight.permute(4, 3, 2, 1, 0) if not transposed else weight.permute(4, 3, 2, 0, 1)
+    return w.reshape(-1, w.shape[3], w.shape[4]).contiguous()
+
+
+def kio_to_weight(gk: torch.Tensor, kernel_size, transposed: bool) -> torch.Tensor:
+    g = gk.reshape(int(kernel_size[2]), int(kernel_size[1]), int(kernel_size[0]), gk.shape[1], gk.shape[2])
+    return g.permute(4, 3, 2, 1, 0) if not transposed else g.permute(3, 4, 2, 1, 0)
+
+
+def wgrad_chunk(n_pairs: int) -> int:
+    """Pairs per split-K chunk: a few thousand chunks per launch, multiples of 64."""
+    c = -(-max(n_pairs, 1) // 4096)
+    c = -(-c // 64) * 64
+    return int(min(16384, max(256, c)))
+
+
+def check_conv_args(feats: torch.Tensor, weight: torch.Tensor, n_in: int, transposed: bool) -> None:
+    """Input checks of ``autograd/SparseConvolutionKernelMap.cpp:34-93``."""
+    if not feats.is_floating_point():
+        raise TypeError("features must have a floating point type")
+    if feats.dim() != 2:
+        raise ValueError(f"Expected features to have 2 dimensions (shape (n, nF)) but got {feats.dim()} dimensions")
+    if not weight.is_floating_point():
+        raise TypeError("kernels must have a floating point type")
+    for i in range(weight.dim()):
+        if weight.shape[i] == 0:
+            raise ValueError(f"kernels tensor has zero dimension (dim = {i})")
+    if feats.shape[0] != n_in:
+        raise ValueError("The number of input features must match the number of voxels")
+    if weight.dim() != 5:
+        raise ValueError(f"Expected kernels to have 5 dimensions but got {weight.dim()} dimensions")
+    cin = weight.shape[0] if transposed else weight.shape[1]
+    if cin != feats.shape[1]:
+        raise ValueError(f"Expected input channels of kernels ({cin}) to equal input channels of features: "
+                         f"{feats.shape[1]}")
+    if weight.dtype != feats.dtype:
+        raise TypeError(f"features ({feats.dtype}) and kernels ({weight.dtype}) must have the same dtype")
+    require_device(feats, "features")
+
+
+class SparseConvFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, feats, weight, pack, transposed: bool):
+        n_src, n_tgt = pack.source_grid.total_voxels, pack.target_grid.total_voxels
+        kio = weight_to_kio(weight, transposed)  # [K, I, O]
+        if not transposed:
+            fmap, n_out = pack._map_fwd(), n_tgt
+        else:
+            fmap, n_out = pack._map_bwd(), n_src
+        out = ext.implicit_gemm(feats.contiguous(), kio.transpose(1, 2).contiguous(), fmap, n_out)
+        ctx.save_for_backward(feats, kio)
+        ctx.pack = pack
+        ctx.transposed = transposed
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        feats, kio = ctx.saved_tensors
+        pack, transposed = ctx.pack, ctx.transposed
+        grad_out = grad_out.contiguous()
+        if grad_out.dtype != feats.dtype:
+            grad_out = grad_out.to(feats.dtype)
+        gin = gw = None
+        if ctx.needs_input_grad[0]:
+            if not transposed:
+                dmap, n_in = pack._map_bwd(), pack.source_grid.total_voxels
+            else:
+                dmap, n_in = pack._map_fwd(), pack.target_grid.total_voxels
+            gin = ext.implicit_gemm(grad_out, kio, dmap, n_in)
+        if ctx.needs_input_grad[1]:
+            pairs, sizes = pack._pairs()
+            gk = ext.wgrad(feats.contiguous(), grad_out, pairs, sizes, transposed, wgrad_chunk(pairs.shape[0]))
+            gw = kio_to_weight(gk.to(kio.dtype), pack.kernel_size, transposed)
+        return gin, gw, None, None
+
+
+def sparse_conv(feats: torch.Tensor, weight: torch.Tensor, pack, transposed: bool) -> torch.Tensor:
+    n_in = pack.target_grid.total_voxels if transposed else pack.source_grid.total_voxels
+    check_conv_args(feats, weight, n_in, transposed)
+    return SparseConvFunction.apply(feats, weight, pack, transposed)

@@ -1,0 +1,483 @@
+"""GridBatch: a batch of sparse voxel grids resident on a HIP device.
+
+Mirrors the convolution-path surface of the reference ``fvdb/grid_batch.py``
+(constructors ``from_ijk`` ``:202``, ``from_points`` ``:304``, ``from_dense`` ``:109``;
+``conv_grid`` ``:515-531``, ``sparse_conv_kernel_map`` ``:1392-1424``,
+``sparse_conv_halo`` ``:1373-1390``, ``jagged_like`` ``:1022``, ``is_same`` ``:1008``,
+``ijk`` ``:1956``, ``write_to_dense_xyzc`` ``:1699`` and neighbours).  The index
+behind it is the device-resident leaf table and hash of ``include/fvdbk.h``
+(``fvdbk_grid``) instead of a NanoVDB grid; voxels are numbered in the same NanoVDB
+tree order.  The target grid of a strided conv keeps the source's voxel sizes and
+origins (``GridBatchImpl.cu:1811-1829``).
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Optional, Sequence, Tuple, Union
+
+import torch
+
+from ._lib import ext, require_device
+from .jagged_tensor import JaggedTensor
+from .types import resolve_device, to_vec3f_batch, to_vec3i
+
+MAX_GRIDS_PER_BATCH = 1024
+
+
+class _GridIndex:
+    """Device tensors of one built grid batch (the fvdbk_grid of the C ABI)."""
+
+    __slots__ = ("ijk", "jidx", "keys", "masks", "wbase", "hash", "counts", "joffsets", "n_grids", "voxel_sizes",
+                 "origins", "device")
+
+    def __init__(self, ijk, jidx, counts_dev, keys, masks, wbase, hash_, n_grids, voxel_sizes, origins):
+        self.ijk = ijk
+        self.jidx = jidx
+        self.keys = keys
+        self.masks = masks
+        self.wbase = wbase
+        self.hash = hash_
+        self.n_grids = int(n_grids)
+        self.device = ijk.device
+        counts = counts_dev.to("cpu", torch.int64)
+        self.counts = counts
+        jo = torch.zeros(self.n_grids + 1, dtype=torch.int64)
+        if self.n_grids > 0:
+            jo[1:] = torch.cumsum(counts, 0)
+        self.joffsets = jo.to(self.device)
+        self.voxel_sizes = voxel_sizes  # float64 CPU [B,3]
+        self.origins = origins
+
+    def tensors(self) -> List[torch.Tensor]:
+        return [self.ijk, self.keys, self.masks, self.wbase, self.hash]
+
+    @property
+    def total_voxels(self) -> int:
+        return int(self.ijk.shape[0])
+
+
+def _build_index(ijk: torch.Tensor, jidx: Optional[torch.Tensor], n_grids: int, voxel_sizes: torch.Tensor,
+                 origins: torch.Tensor) -> _GridIndex:
+    require_device(ijk, "ijk")
+    if n_grids > MAX_GRIDS_PER_BATCH:
+        raise ValueError(f"a GridBatch holds at most {MAX_GRIDS_PER_BATCH} grids")
+    ijk = ijk.to(torch.int32).contiguous()
+    j = jidx.to(device=ijk.device, dtype=torch.int32).contiguous() if jidx is not None else \
+        torch.empty(0, dtype=torch.int32, device=ijk.device)
+    out = ext.build_grid(ijk, j, int(n_grids))
+    ijk_o, jidx_o, counts, keys, masks, wbase, hash_ = out
+    return _GridIndex(ijk_o, jidx_o, counts, keys, masks, wbase, hash_, n_grids, voxel_sizes, origins)
+
+
+def _as_jagged(x) -> JaggedTensor:
+    if isinstance(x, JaggedTensor):
+        return x
+    if isinstance(x, torch.Tensor):
+        return JaggedTensor(x)
+    if isinstance(x, (list, tuple)):
+        return JaggedTensor(list(x))
+    raise TypeError(f"expected a JaggedTensor or torch.Tensor, got {type(x)}")
+
+
+def _full_jidx(j: JaggedTensor) -> torch.Tensor:
+    if j.num_tensors <= 1:
+        return torch.zeros(j.element_count, dtype=torch.int32, device=j.device)
+    return j.jidx.to(torch.int32)
+
+
+class GridBatch:
+    def __init__(self, *, impl: _GridIndex):
+        self._impl = impl
+        self._halo_pack = None
+
+    # ----------------------------------------------------------------------------
+    # constructors
+    # ----------------------------------------------------------------------------
+    @classmethod
+    def from_ijk(cls, ijk, voxel_sizes=1, origins=0, device=None) -> "GridBatch":
+        j = _as_jagged(ijk)
+        dev = resolve_device(device, inherit_from=j.jdata)
+        B = j.num_tensors
+        vs = to_vec3f_batch(voxel_sizes, B, "voxel_sizes", positive=True)
+        og = to_vec3f_batch(origins, B, "origins")
+        data = j.jdata.to(dev)
+        if data.dim() != 2 or data.shape[1] != 3:
+            raise ValueError("ijk must have shape [N, 3]")
+        if data.is_floating_point():
+            raise TypeError("ijk must be an integer tensor")
+        return cls(impl=_build_index(data, _full_jidx(j).to(dev), B, vs, og))
+
+    @classmethod
+    def from_points(cls, points, voxel_sizes=1, origins=0, device=None) -> "GridBatch":
+        j = _as_jagged(points)
+        dev = resolve_device(device, inherit_from=j.jdata)
+        B = j.num_tensors
+        vs = to_vec3f_batch(voxel_sizes, B, "voxel_sizes", positive=True)
+        og = to_vec3f_batch(origins, B, "origins")
+        pts = j.jdata.to(dev)
+        if pts.dim() != 2 or pts.shape[1] != 3:
+            raise ValueError("points must have shape [N, 3]")
+        require_device(pts, "points")
+        if pts.dtype not in (torch.float32, torch.float64):
+            pts = pts.float()
+        # ijk = round(p * (1/vs) + (-origin/vs))  (VoxelCoordTransform.h:296-307)
+        xf = torch.cat([1.0 / vs, -og / vs], 1).to(dev)
+        jid = _full_jidx(j).to(dev)
+        ijk = ext.points_to_ijk(pts.contiguous(), jid, xf)
+        return cls(impl=_build_index(ijk, jid, B, vs, og))
+
+    @classmethod
+    def from_dense(cls, num_grids: int, dense_dims, ijk_min=0, voxel_sizes=1, origins=0, mask=None,
+                   device=None) -> "GridBatch":
+        dev = resolve_device(device, inherit_from=mask)
+        dims = to_vec3i(dense_dims, "dense_dims", positive=True)
+        lo = to_vec3i(ijk_min, "ijk_min")
+        vs = to_vec3f_batch(voxel_sizes, num_grids, "voxel_sizes", positive=True)
+        og = to_vec3f_batch(origins, num_grids, "origins")
+        ax = [torch.arange(lo[d], lo[d] + dims[d], dtype=torch.int32, device=dev) for d in range(3)]
+        g = torch.stack(torch.meshgrid(*ax, indexing="ij"), -1)
+        if mask is not None:
+            m = mask.to(dev).bool()
+            if tuple(m.shape) != tuple(dims):
+                raise ValueError("mask must have the dense dimensions")
+            g = g[m]
+        g = g.reshape(-1, 3)
+        n = g.shape[0]
+        ijk = g.repeat(num_grids, 1)
+        jidx = torch.arange(num_grids, dtype=torch.int32, device=dev).repeat_interleave(n)
+        return cls(impl=_build_index(ijk, jidx, num_grids, vs, og))
+
+    @classmethod
+    def from_zero_voxels(cls, device="cuda", voxel_sizes=1, origins=0) -> "GridBatch":
+        dev = resolve_device(device)
+        vs = to_vec3f_batch(voxel_sizes, 1, "voxel_sizes", positive=True)
+        og = to_vec3f_batch(origins, 1, "origins")
+        return cls(impl=_build_index(torch.empty((0, 3), dtype=torch.int32, device=dev), None, 1, vs, og))
+
+    # ----------------------------------------------------------------------------
+    # properties
+    # ----------------------------------------------------------------------------
+    @property
+    def address(self) -> int:
+        return id(self._impl)
+
+    @property
+    def device(self) -> torch.device:
+        return self._impl.device
+
+    @property
+    def grid_count(self) -> int:
+        return self._impl.n_grids
+
+    def __len__(self) -> int:
+        return self._impl.n_grids
+
+    @property
+    def total_voxels(self) -> int:
+        return self._impl.total_voxels
+
+    @property
+    def num_voxels(self) -> torch.Tensor:
+        return self._impl.counts.clone()
+
+    def num_voxels_at(self, bi: int) -> int:
+        return int(self._impl.counts[bi])
+
+    @property
+    def cum_voxels(self) -> torch.Tensor:
+        return self._impl.joffsets[:-1].clone()
+
+    def cum_voxels_at(self, bi: int) -> int:
+        return int(self._impl.joffsets[bi].item())
+
+    @property
+    def joffsets(self) -> torch.Tensor:
+        return self._impl.joffsets
+
+    @property
+    def jidx(self) -> torch.Tensor:
+        if self.grid_count <= 1:
+            return torch.empty(0, dtype=torch.int32, device=self.device)
+        return self._impl.jidx
+
+    @property
+    def ijk(self) -> JaggedTensor:
+        return self.jagged_like(self._impl.ijk)
+
+    @property
+    def voxel_sizes(self) -> torch.Tensor:
+        return self._impl.voxel_sizes.to(self.device, torch.float32)
+
+    @property
+    def origins(self) -> torch.Tensor:
+        return self._impl.origins.to(self.device, torch.float32)
+
+    def voxel_size_at(self, bi: int) -> torch.Tensor:
+        return self._impl.voxel_sizes[bi].to(self.device, torch.float32)
+
+    def origin_at(self, bi: int) -> torch.Tensor:
+        return self._impl.origins[bi].to(self.device, torch.float32)
+
+    @property
+    def num_leaf_nodes(self) -> torch.Tensor:
+        b = (self._impl.keys >> 51).long()
+        return torch.bincount(b, minlength=self.grid_count).cpu()
+
+    @property
+    def total_leaf_nodes(self) -> int:
+        return int(self._impl.keys.shape[0])
+
+    @property
+    def total_bytes(self) -> int:
+        return int(sum(t.numel() * t.element_size() for t in self._impl.tensors()) +
+                   self._impl.jidx.numel() * 4)
+
+    @property
+    def bboxes(self) -> torch.Tensor:
+        out = torch.zeros((self.grid_count, 2, 3), dtype=torch.int32)
+        o = self._impl.joffsets.cpu().tolist()
+        for b in range(self.grid_count):
+            if o[b + 1] > o[b]:
+                part = self._impl.ijk[o[b]:o[b + 1]]
+                out[b, 0] = part.min(0).values.cpu()
+                out[b, 1] = part.max(0).values.cpu()
+        return out.to(self.device)
+
+    def bbox_at(self, bi: int) -> torch.Tensor:
+        return self.bboxes[bi]
+
+    @property
+    def total_bbox(self) -> torch.Tensor:
+        if self.total_voxels == 0:
+            return torch.zeros((2, 3), dtype=torch.int32, device=self.device)
+        return torch.stack([self._impl.ijk.min(0).values, self._impl.ijk.max(0).values])
+
+    def has_zero_voxels_at(self, bi: int) -> bool:
+        return int(self._impl.counts[bi]) == 0
+
+    @property
+    def all_have_zero_voxels(self) -> bool:
+        return self.total_voxels == 0
+
+    @property
+    def any_have_zero_voxels(self) -> bool:
+        return bool((self._impl.counts == 0).any())
+
+    def is_same(self, other: "GridBatch") -> bool:
+        """Identity of the underlying index, as in ``GridBatch.cpp:79-81``."""
+        return isinstance(other, GridBatch) and other._impl is self._impl
+
+    def has_same_address_and_grid_count(self, other) -> bool:
+        if isinstance(other, GridBatch):
+            return other._impl is self._impl and other.grid_count == self.grid_count
+        return False
+
+    def jagged_like(self, data: torch.Tensor) -> JaggedTensor:
+        if data.shape[0] != self.total_voxels:
+            raise ValueError(f"data has {data.shape[0]} rows, the grid has {self.total_voxels} voxels")
+        jidx = self._impl.jidx if self.grid_count > 1 else torch.empty(0, dtype=torch.int32, device=data.device)
+        return JaggedTensor(_jdata=data, _joffsets=self._impl.joffsets.to(data.device), _jidx=jidx.to(data.device))
+
+    # ----------------------------------------------------------------------------
+    # derived grids
+    # ----------------------------------------------------------------------------
+    def _derived(self, cand_ijk: torch.Tensor, cand_jidx: torch.Tensor, voxel_sizes=None,
+                 origins=None) -> "GridBatch":
+        vs = self._impl.voxel_sizes if voxel_sizes is None else voxel_sizes
+        og = self._impl.origins if origins is None else origins
+        return GridBatch(impl=_build_index(cand_ijk, cand_jidx, self.grid_count, vs.clone(), og.clone()))
+
+    def conv_grid(self, kernel_size, stride=0) -> "GridBatch":
+        """Target grid of a strided convolution (``BuildGridForConv.cu:126-255``)."""
+        ks = to_vec3i(kernel_size, "kernel_size", positive=True)
+        st = to_vec3i(stride, "stride", non_negative=True)
+        if any(s <= 0 for s in st):
+            raise ValueError("stride must be strictly positive.")
+        cijk, cj = ext.conv_grid_candidates(self._impl.ijk, self._impl.jidx, list(ks), list(st))
+        return self._derived(cijk, cj)
+
+    def coarsened_grid(self, coarsening_factor) -> "GridBatch":
+        """Voxels {floor(ijk / factor)} with voxel size scaled by the factor."""
+        f = to_vec3i(coarsening_factor, "coarsening_factor", positive=True)
+        # coarse mode: floor(ijk / factor) with the factor indexed by coordinate axis
+        cijk, cj = ext.conv_grid_candidates(self._impl.ijk, self._impl.jidx, list(f), list(f))
+        fac = torch.tensor(f, dtype=torch.float64)
+        vs = self._impl.voxel_sizes * fac
+        og = self._impl.origins + 0.5 * (fac - 1) * self._impl.voxel_sizes
+        return self._derived(cijk, cj, vs, og)
+
+    def subdivided_grid(self, subdiv_factor, mask: Optional[JaggedTensor] = None) -> "GridBatch":
+        """Each voxel split into factor^3 children (the fine grid of a coarse one)."""
+        f = to_vec3i(subdiv_factor, "subdiv_factor", positive=True)
+        ijk, jidx = self._impl.ijk, self._impl.jidx
+        if mask is not None:
+            m = (mask.jdata if isinstance(mask, JaggedTensor) else mask).bool()
+            ijk, jidx = ijk[m], jidx[m]
+        off = torch.stack(torch.meshgrid(*[torch.arange(f[d], device=self.device, dtype=torch.int32)
+                                           for d in range(3)], indexing="ij"), -1).reshape(-1, 3)
+        ft = torch.tensor(f, dtype=torch.int32, device=self.device)
+        child = (ijk[:, None, :] * ft + off[None]).reshape(-1, 3)
+        cj = jidx.repeat_interleave(off.shape[0])
+        fac = torch.tensor(f, dtype=torch.float64)
+        vs = self._impl.voxel_sizes / fac
+        og = self._impl.origins - 0.5 * (1 - 1 / fac) * self._impl.voxel_sizes
+        return self._derived(child, cj, vs, og)
+
+    def __getitem__(self, index) -> "GridBatch":
+        B = self.grid_count
+        if isinstance(index, int):
+            if index < 0:
+                index += B
+            if not 0 <= index < B:
+                raise IndexError("grid index out of range")
+            sel = [index]
+        elif isinstance(index, slice):
+            sel = list(range(B))[index]
+        elif isinstance(index, (list, tuple)):
+            sel = [i for i, v in enumerate(index) if v] if all(isinstance(v, bool) for v in index) else list(index)
+        elif isinstance(index, torch.Tensor):
+            sel = index.nonzero().reshape(-1).tolist() if index.dtype == torch.bool else index.reshape(-1).tolist()
+        else:
+            raise TypeError(f"unsupported index type {type(index)}")
+        o = self._impl.joffsets.cpu().tolist()
+        parts, jids = [], []
+        for new_b, b in enumerate(sel):
+            parts.append(self._impl.ijk[o[b]:o[b + 1]])
+            jids.append(torch.full((o[b + 1] - o[b],), new_b, dtype=torch.int32, device=self.device))
+        ijk = torch.cat(parts) if parts else torch.empty((0, 3), dtype=torch.int32, device=self.device)
+        jid = torch.cat(jids) if jids else torch.empty(0, dtype=torch.int32, device=self.device)
+        return GridBatch(impl=_build_index(ijk, jid, len(sel), self._impl.voxel_sizes[sel].clone(),
+                                           self._impl.origins[sel].clone()))
+
+    def __iter__(self):
+        for b in range(self.grid_count):
+            yield self[b]
+
+    def to(self, target) -> "GridBatch":
+        dev = target.device if hasattr(target, "device") else torch.device(target)
+        if dev.type != "cuda":
+            raise ValueError("fvdb-core_amd grids live on HIP devices only")
+        if dev == self.device:
+            return self
+        return GridBatch(impl=_build_index(self._impl.ijk.to(dev), self._impl.jidx.to(dev), self.grid_count,
+                                           self._impl.voxel_sizes, self._impl.origins))
+
+    def cuda(self) -> "GridBatch":
+        return self.to("cuda")
+
+    def cpu(self) -> "GridBatch":
+        raise ValueError("fvdb-core_amd grids live on HIP devices only")
+
+    # ----------------------------------------------------------------------------
+    # queries
+    # ----------------------------------------------------------------------------
+    def ijk_to_index(self, ijk, cumulative: bool = False) -> JaggedTensor:
+        j = _as_jagged(ijk)
+        if j.num_tensors != self.grid_count:
+            raise ValueError("ijk must have one list per grid")
+        jid = _full_jidx(j).to(self.device)
+        idx = ext.ijk_to_index(*self._impl.tensors(), self.grid_count, j.jdata.to(self.device), jid).long()
+        if not cumulative:
+            base = self._impl.joffsets[jid.long()]
+            idx = torch.where(idx >= 0, idx - base, idx)
+        return j.jagged_like(idx)
+
+    def coords_in_active_voxel(self, ijk) -> JaggedTensor:
+        idx = self.ijk_to_index(ijk, cumulative=True)
+        return idx.jagged_like(idx.jdata >= 0)
+
+    # ----------------------------------------------------------------------------
+    # dense I/O (differentiable; used by the dense-equivalence tests)
+    # ----------------------------------------------------------------------------
+    def _dense_frame(self, min_coord, grid_size):
+        if min_coord is None:
+            lo = self.total_bbox[0].to(torch.int64).reshape(1, 3).expand(self.grid_count, 3)
+        else:
+            lo = torch.as_tensor(min_coord, dtype=torch.int64, device=self.device).reshape(-1, 3)
+            lo = lo.expand(self.grid_count, 3)
+        if grid_size is None:
+            hi = self.total_bbox[1].to(torch.int64)
+            size = (hi - lo.min(0).values + 1).clamp(min=0)
+            size = tuple(int(v) for v in size.tolist())
+        else:
+            size = to_vec3i(grid_size, "grid_size", positive=True)
+        return lo, size
+
+    def write_to_dense_xyzc(self, sparse_data, min_coord=None, grid_size=None) -> torch.Tensor:
+        data = sparse_data.jdata if isinstance(sparse_data, JaggedTensor) else sparse_data
+        if data.shape[0] != self.total_voxels:
+            raise ValueError("sparse_data must have one row per voxel")
+        lo, size = self._dense_frame(min_coord, grid_size)
+        b = self._impl.jidx.long()
+        rel = self._impl.ijk.long() - lo[b]
+        inside = ((rel >= 0) & (rel < torch.tensor(size, device=self.device))).all(1)
+        out = torch.zeros((self.grid_count,) + size + tuple(data.shape[1:]), dtype=data.dtype, device=data.device)
+        sel = inside.nonzero().reshape(-1)
+        r = rel[sel]
+        return out.index_put((b[sel], r[:, 0], r[:, 1], r[:, 2]), data[sel])
+
+    def write_to_dense_czyx(self, sparse_data, min_coord=None, grid_size=None) -> torch.Tensor:
+        d = self.write_to_dense_xyzc(sparse_data, min_coord, grid_size)
+        if d.dim() == 4:
+            d = d.unsqueeze(-1)
+        return d.permute(0, 4, 3, 2, 1)
+
+    def read_from_dense_xyzc(self, dense_data: torch.Tensor, dense_origins=0) -> JaggedTensor:
+        lo = torch.as_tensor(dense_origins, dtype=torch.int64, device=self.device).reshape(-1, 3)
+        lo = lo.expand(self.grid_count, 3)
+        b = self._impl.jidx.long()
+        rel = self._impl.ijk.long() - lo[b]
+        size = torch.tensor(dense_data.shape[1:4], device=self.device)
+        inside = ((rel >= 0) & (rel < size)).all(1)
+        out = torch.zeros((self.total_voxels,) + tuple(dense_data.shape[4:]), dtype=dense_data.dtype,
+                          device=dense_data.device)
+        sel = inside.nonzero().reshape(-1)
+        r = rel[sel]
+        out = out.index_put((sel,), dense_data[b[sel], r[:, 0], r[:, 1], r[:, 2]])
+        return self.jagged_like(out)
+
+    def read_from_dense_czyx(self, dense_data: torch.Tensor, dense_origins=0) -> JaggedTensor:
+        return self.read_from_dense_xyzc(dense_data.permute(0, 4, 3, 2, 1), dense_origins)
+
+    # ----------------------------------------------------------------------------
+    # convolution entry points
+    # ----------------------------------------------------------------------------
+    def sparse_conv_kernel_map(self, kernel_size, stride, target_grid: Optional["GridBatch"] = None):
+        """(SparseConvPackInfo, target grid) -- ``GridBatchBinding.cpp:858-868``."""
+        from .sparse_conv_pack_info import SparseConvPackInfo
+
+        ks = to_vec3i(kernel_size, "kernel_size", positive=True)
+        st = to_vec3i(stride, "stride", non_negative=True)
+        pack = SparseConvPackInfo(ks, st, self, target_grid)
+        return pack, pack.target_grid
+
+    def sparse_conv_halo(self, input, weight: torch.Tensor, variant: int = 8) -> JaggedTensor:
+        """Stride-1 3x3x3 convolution on this grid (``GridBatch.cpp:775-792``).
+
+        Served by the same gfx950 implicit-GEMM kernel as the other backends (the
+        reference's halo kernel is a separate TF32 stencil); ``variant`` is accepted for
+        API compatibility.
+        """
+        from ._autograd import sparse_conv
+        from .sparse_conv_pack_info import SparseConvPackInfo
+
+        j = _as_jagged(input)
+        if j.ldim != 1:
+            raise ValueError("Expected input to have 1 list dimension")
+        if not j.is_floating_point():
+            raise TypeError("input must have a floating point type")
+        if j.rsize(0) != self.total_voxels:
+            raise ValueError("Value count not match!")
+        if j.num_tensors != self.grid_count:
+            raise ValueError("Batch size not match!")
+        if variant not in (8, 64):
+            raise ValueError("variant must be 8 or 64")
+        if tuple(weight.shape[2:]) != (3, 3, 3):
+            raise ValueError("halo convolution needs a 3x3x3 kernel")
+        if self._halo_pack is None:
+            self._halo_pack = SparseConvPackInfo((3, 3, 3), (1, 1, 1), self, None)
+        out = sparse_conv(j.jdata, weight, self._halo_pack, False)
+        return j.jagged_like(out)
+
+    def __repr__(self) -> str:
+        return f"GridBatch(grid_count={self.grid_count}, total_voxels={self.total_voxels}, device={self.device})"

@@ -1,0 +1,7 @@
+"""Neural-network layers on sparse voxel grids (the convolution path of ``fvdb.nn``)."""
+from .modules import (BatchNorm, Dropout, ElementwiseMixin, GELU, GroupNorm, LeakyReLU, Linear, ReLU, Sigmoid, SiLU,
+                      SparseConv3d, Tanh, fvnn_module)
+from .vdbtensor import VDBTensor, vdbtensor_from_dense
+
+__all__ = ["SparseConv3d", "VDBTensor", "vdbtensor_from_dense", "ReLU", "LeakyReLU", "SiLU", "GELU", "Sigmoid",
+           "Tanh", "Linear", "Dropout", "BatchNorm", "GroupNorm", "ElementwiseMixin", "fvnn_module"]

@@ -1,0 +1,107 @@
+"""Seeded synthetic sparse grids for the benchmark workloads.
+
+* ``sphere_points``: points sampled uniformly on random spheres (centres U[0,1]^3,
+  radii U[0.1, 0.3]) -- the single-grid and batched sphere workloads.
+* ``planes_spheres_points``: random planes plus spheres inside the unit cube.
+* ``indoor_ijk``: axis-aligned room walls/floor plus random boxes (surfaces only),
+  voxelised directly on a 2 cm lattice of a 10 x 10 x 3 m room.
+
+Points are generated on the device with a seeded ``torch.Generator`` and voxelised by
+``GridBatch.from_points`` (``ijk = round(p / voxel_size)``).
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Sequence, Tuple
+
+import torch
+
+
+def _gen(seed: int, device) -> torch.Generator:
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    return g
+
+
+def sphere_points(seed: int, n_points: int = 2_000_000, n_spheres: int = 8, device="cuda") -> torch.Tensor:
+    g = _gen(seed, device)
+    c = torch.rand((n_spheres, 3), generator=g, device=device, dtype=torch.float64)
+    r = 0.1 + 0.2 * torch.rand((n_spheres,), generator=g, device=device, dtype=torch.float64)
+    per = n_points // n_spheres
+    d = torch.randn((n_spheres, per, 3), generator=g, device=device, dtype=torch.float64)
+    d = d / d.norm(dim=-1, keepdim=True)
+    p = c[:, None, :] + r[:, None, None] * d
+    return p.reshape(-1, 3).float()
+
+
+def planes_spheres_points(seed: int, n_points: int = 8_000_000, n_planes: int = 4, n_spheres: int = 4,
+                          device="cuda") -> torch.Tensor:
+    g = _gen(seed, device)
+    per = n_points // (n_planes + n_spheres)
+    parts = []
+    for _ in range(n_planes):
+        n = torch.randn(3, generator=g, device=device, dtype=torch.float64)
+        n = n / n.norm()
+        o = torch.rand(3, generator=g, device=device, dtype=torch.float64)
+        a = torch.linalg.cross(n, torch.tensor([0.0, 0.0, 1.0], device=device, dtype=torch.float64))
+        if a.norm() < 1e-3:
+            a = torch.tensor([1.0, 0.0, 0.0], device=device, dtype=torch.float64)
+        a = a / a.norm()
+        b = torch.linalg.cross(n, a)
+        uv = torch.rand((per, 2), generator=g, device=device, dtype=torch.float64) - 0.5
+        p = o + uv[:, :1] * a + uv[:, 1:] * b
+        parts.append(p[((p >= 0) & (p <= 1)).all(1)])
+    for _ in range(n_spheres):
+        c = torch.rand(3, generator=g, device=device, dtype=torch.float64)
+        r = 0.1 + 0.2 * torch.rand((), generator=g, device=device, dtype=torch.float64)
+        d = torch.randn((per, 3), generator=g, device=device, dtype=torch.float64)
+        parts.append(c + r * d / d.norm(dim=-1, keepdim=True))
+    return torch.cat(parts).float()
+
+
+def indoor_ijk(seed: int, n_boxes: int = 20, room=(500, 500, 150), device="cuda") -> torch.Tensor:
+    """Surface voxels of a room (floor, ceiling, 4 walls) and random boxes, 2 cm lattice."""
+    g = _gen(seed, device)
+    X, Y, Z = room
+    faces: List[torch.Tensor] = []
+
+    def rect(axis: int, at: int, lo: Sequence[int], hi: Sequence[int]):
+        # all lattice cells of an axis-aligned rectangle at coordinate `at` along `axis`
+        dims = [d for d in range(3) if d != axis]
+        a = torch.arange(lo[0], hi[0] + 1, device=device, dtype=torch.int32)
+        b = torch.arange(lo[1], hi[1] + 1, device=device, dtype=torch.int32)
+        A, B = torch.meshgrid(a, b, indexing="ij")
+        out = torch.empty((A.numel(), 3), dtype=torch.int32, device=device)
+        out[:, axis] = at
+        out[:, dims[0]] = A.reshape(-1)
+        out[:, dims[1]] = B.reshape(-1)
+        faces.append(out)
+
+    def box(lo, hi):
+        for axis in range(3):
+            dims = [d for d in range(3) if d != axis]
+            for at in (lo[axis], hi[axis]):
+                rect(axis, at, [lo[dims[0]], lo[dims[1]]], [hi[dims[0]], hi[dims[1]]])
+
+    # room shell (open at the top half of one wall to vary topology a little)
+    rect(2, 0, [0, 0], [X - 1, Y - 1])
+    rect(2, Z - 1, [0, 0], [X - 1, Y - 1])
+    rect(0, 0, [0, 0], [Y - 1, Z - 1])
+    rect(0, X - 1, [0, 0], [Y - 1, Z - 1])
+    rect(1, 0, [0, 0], [X - 1, Z - 1])
+    rect(1, Y - 1, [0, 0], [X - 1, Z // 2])
+    for _ in range(n_boxes):
+        size = (torch.rand(3, generator=g, device=device) * torch.tensor([120.0, 120.0, 100.0], device=device)
+                + 10).int()
+        lo = (torch.rand(3, generator=g, device=device) * (torch.tensor([X, Y, Z], device=device) - size)).int()
+        lo[2] = 0 if bool(torch.rand((), generator=g, device=device) < 0.6) else lo[2]
+        hi = lo + size
+        hi = torch.minimum(hi, torch.tensor([X - 1, Y - 1, Z - 1], device=device, dtype=torch.int32))
+        box(lo.tolist(), hi.tolist())
+    return torch.cat(faces)
+
+
+def log_uniform(seed: int, lo: float, hi: float, n: int) -> List[float]:
+    g = torch.Generator().manual_seed(int(seed))
+    u = torch.rand(n, generator=g, dtype=torch.float64)
+    return torch.exp(math.log(lo) + u * (math.log(hi) - math.log(lo))).tolist()

@@ -1,0 +1,220 @@
+/*
+ * fvdbk — C ABI of the MI355X (gfx950) sparse-convolution kernels.
+ *
+ * Plain C: no torch types, only device pointers, sizes and a hipStream_t.  The
+ * caller owns every buffer (inputs, outputs, workspaces) and the stream; no entry
+ * point allocates device memory, synchronises the stream or reads device memory
+ * on the host.  Every entry point returns FVDBK_OK (0) or a negative status and
+ * records a message retrievable with fvdbk_last_error() (thread-local).  Status
+ * classes mirror the reference's error classes: FVDBK_EINVAL is what the reference
+ * raises with TORCH_CHECK_VALUE (Python ValueError), FVDBK_ETYPE with
+ * TORCH_CHECK_TYPE (TypeError).
+ *
+ * Each declaration names the reference interface it replaces.  Reference paths are
+ * relative to the fvdb-core repository root.
+ */
+#ifndef FVDBK_H
+#define FVDBK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum fvdbk_status {
+    FVDBK_OK = 0,
+    FVDBK_EINVAL = -1,      /* bad argument value: ValueError */
+    FVDBK_ETYPE = -2,       /* unsupported dtype: TypeError */
+    FVDBK_EHIP = -3,        /* HIP launch/runtime error */
+    FVDBK_ENOSPC = -4,      /* workspace or LDS too small */
+    FVDBK_EUNSUPPORTED = -5 /* valid but not implemented */
+};
+
+typedef enum fvdbk_dtype {
+    FVDBK_F32 = 0,
+    FVDBK_F64 = 1,
+    FVDBK_F16 = 2,
+    FVDBK_BF16 = 3
+} fvdbk_dtype;
+
+/* One open-addressing slot of a grid's leaf hash: key is the leaf's sort key,
+ * leaf is its index in the sorted leaf arrays, -1 when the slot is empty. */
+typedef struct fvdbk_hslot {
+    uint64_t key;
+    int32_t leaf;
+    int32_t pad;
+} fvdbk_hslot;
+
+/*
+ * Device-resident index of a batch of sparse grids (replaces the NanoVDB OnIndex
+ * grid batch of src/fvdb/detail/GridBatchImpl.h:31-231 for the convolution path).
+ *
+ * Voxels are numbered in NanoVDB tree order: grid, then root tile, upper-node
+ * child, lower-node child, leaf bit.  A leaf is an 8^3 brick with a 512-bit
+ * activity mask; its sort key packs (grid, tile, upper child, lower child) so
+ * that sorting keys sorts leaves in tree order.  leaf_word_base[l][w] is the
+ * global index of the first active voxel of 64-bit word w of leaf l, so the index
+ * of bit b of leaf l is leaf_word_base[l][b>>6] + popcount(mask[l][b>>6] & ((1<<(b&63))-1)).
+ */
+typedef struct fvdbk_grid {
+    int64_t n_voxels;
+    int64_t n_leaves;
+    int64_t hash_cap;                /* power of two */
+    int32_t n_grids;
+    int32_t pad;
+    const int32_t* ijk;              /* [n_voxels][3] in tree order */
+    const uint64_t* leaf_keys;       /* [n_leaves] ascending */
+    const uint64_t* leaf_masks;      /* [n_leaves][8] */
+    const int32_t* leaf_word_base;   /* [n_leaves][8] */
+    const fvdbk_hslot* hash;         /* [hash_cap] */
+} fvdbk_grid;
+
+/* Thread-local message of the last failing call on this thread. */
+const char* fvdbk_last_error(void);
+
+/* Library version string. */
+const char* fvdbk_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Grid index build (replaces nanovdb::tools::cuda::voxelsToGrid as called by
+ * src/fvdb/detail/ops/BuildGridFromIjk.cu:27-84 and the per-voxel ijk/jidx export
+ * of src/fvdb/detail/GridBatchImpl.cu).  A build is a short sequence because the
+ * number of leaves is only known after de-duplicating keys:
+ *   1. fvdbk_grid_leaf_keys     per input voxel: leaf sort key (UINT64_MAX = skip)
+ *   2. caller sorts + de-duplicates the keys (any device sort) -> n_leaves keys
+ *   3. fvdbk_grid_hash_build    leaf hash over the unique keys
+ *   4. fvdbk_grid_leaf_masks    512-bit masks (OR of the input voxels)
+ *   5. fvdbk_grid_leaf_counts   active voxels per leaf
+ *   6. caller exclusive-scans the counts -> leaf_start[n_leaves]
+ *   7. fvdbk_grid_finalize      word bases, ijk and grid index of every voxel
+ * ------------------------------------------------------------------------- */
+
+/* batch_idx may be NULL (all voxels in grid 0); entries < 0 are skipped.
+ * Coordinates must lie in [-2^20, 2^20); *bad_count (device int32) is incremented
+ * per out-of-range voxel (caller zeroes it and checks it). */
+int fvdbk_grid_leaf_keys(const int32_t* ijk, const int32_t* batch_idx, int64_t n, uint64_t* keys,
+                         int32_t* bad_count, hipStream_t stream);
+
+int fvdbk_grid_hash_build(const uint64_t* leaf_keys, int64_t n_leaves, fvdbk_hslot* hash, int64_t hash_cap,
+                          hipStream_t stream);
+
+/* leaf_masks [n_leaves][8] is cleared by the call. */
+int fvdbk_grid_leaf_masks(const int32_t* ijk, const int32_t* batch_idx, int64_t n, const fvdbk_hslot* hash,
+                          int64_t hash_cap, uint64_t* leaf_masks, int64_t n_leaves, hipStream_t stream);
+
+int fvdbk_grid_leaf_counts(const uint64_t* leaf_masks, int64_t n_leaves, int32_t* counts, hipStream_t stream);
+
+/* Writes leaf_word_base [n_leaves][8], ijk_out [n_voxels][3], jidx_out [n_voxels]
+ * (grid index per voxel; may be NULL) and grid_counts [n_grids] (voxels per grid,
+ * cleared by the call; may be NULL). */
+int fvdbk_grid_finalize(const uint64_t* leaf_keys, const uint64_t* leaf_masks, const int64_t* leaf_start,
+                        int64_t n_leaves, int32_t* leaf_word_base, int32_t* ijk_out, int32_t* jidx_out,
+                        int32_t* grid_counts, int32_t n_grids, hipStream_t stream);
+
+/* Linear index of query coordinates (-1 when inactive).  Replaces
+ * src/fvdb/detail/ops/IjkToIndex.cu (GridBatch.ijk_to_index, cumulative indices). */
+int fvdbk_grid_ijk_to_index(const fvdbk_grid* grid, const int32_t* ijk, const int32_t* batch_idx, int64_t n,
+                            int32_t* out_index, hipStream_t stream);
+
+/* World points -> voxel coordinates, ijk = floor(p * scale + translate + 0.5) in the
+ * point precision (src/fvdb/detail/ops/BuildGridFromPoints.cu:28-44 with
+ * VoxelCoordTransform.h:88-107).  transforms is [n_grids][6] (scale xyz, translate
+ * xyz) as double; points are float32 (dtype FVDBK_F32) or float64. */
+int fvdbk_points_to_ijk(fvdbk_dtype dtype, const void* points, const int32_t* batch_idx, int64_t n,
+                        const double* transforms, int32_t* ijk_out, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Convolution target grid (src/fvdb/detail/ops/BuildGridForConv.cu:66-255 and
+ * CoarseIjkForFineGrid.cu:18-111).  Emits n * K' candidate coordinates with their
+ * grid index (-1 = not a target) that step 1 of the grid build consumes;
+ * K' = 1 when stride is all ones or equals the kernel size (floor(ijk/stride)),
+ * else K' = ks[0]*ks[1]*ks[2].  ks/stride are in the reference's (d,h,w) order.
+ * ------------------------------------------------------------------------- */
+int64_t fvdbk_conv_grid_candidates_per_voxel(const int32_t ks[3], const int32_t stride[3]);
+
+int fvdbk_conv_grid_candidates(const int32_t* ijk, const int32_t* jidx, int64_t n, const int32_t ks[3],
+                               const int32_t stride[3], int32_t* cand_ijk, int32_t* cand_jidx,
+                               hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Kernel map (replaces ops::dispatchConvolutionKernelMap<kCUDA>,
+ * src/fvdb/detail/ops/convolution/pack_info/ConvolutionKernelMap.h:16-21 and
+ * ConvolutionKernelMap.cu:104-138).
+ *
+ * inverse == 0: rows = target grid, probe = source grid; map[t][k] = index of
+ *   source voxel t*stride + kernelStart + offset(k), or -1 (the reference's
+ *   out_in_map / kernel map).
+ * inverse == 1: rows = source grid, probe = target grid; map[s][k] = t such that
+ *   the forward map has map[t][k] = s, or -1 (the reference's transposed map,
+ *   IGEMMBitOperations.cu:92-105, computed by probing instead of scattering).
+ * Rows [rows->n_voxels, n_rows_padded) are filled with -1.  ks/stride in (d,h,w).
+ * ------------------------------------------------------------------------- */
+int fvdbk_kmap_build(const fvdbk_grid* rows, const fvdbk_grid* probe, const int32_t ks[3],
+                     const int32_t stride[3], int32_t inverse, int32_t* map, int64_t n_rows_padded,
+                     hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Gather-scatter neighbour map (replaces SparseConvPackInfo::buildGatherScatter,
+ * src/fvdb/SparseConvPackInfo.cpp:61-88): pairs (source index, target row)
+ * ordered by (k, target row), nbsizes[k] = pairs of offset k.
+ *   fvdbk_nbmap_block_counts: counts[b][k] for row blocks of FVDBK_NBMAP_ROWS rows
+ *   caller: exclusive scan of counts in (k, block) order -> block_offsets [nblk][K]
+ *   fvdbk_nbmap_emit: writes pairs [P][2]
+ * ------------------------------------------------------------------------- */
+#define FVDBK_NBMAP_ROWS 1024
+int fvdbk_nbmap_block_counts(const int32_t* map, int64_t n_rows, int32_t K, int32_t* counts, hipStream_t stream);
+int fvdbk_nbmap_emit(const int32_t* map, int64_t n_rows, int32_t K, const int64_t* block_offsets, int32_t* pairs,
+                     hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Implicit-GEMM contraction (replaces dispatchSparseConvolutionImplicitGEMM /
+ * ...Sorted, src/fvdb/detail/ops/convolution/backend/SparseConvolutionImplicitGEMM.h
+ * and SparseConvolutionImplicitGEMMSorted.h, and the forward of
+ * dispatchSparseConvolutionKernelMap, backend/SparseConvolutionKernelMap.h:13-21).
+ *
+ *   out[r][m] = sum_k sum_c  in[map[r][k]][c] * a[k][m][c]      (map entries < 0 skipped)
+ *
+ * in  [n_in][c_in], a [K][M][c_in], map [n_rows][K] (row stride K), out [n_rows][M],
+ * all of dtype `dtype`; accumulation in fp32 (fp64 for FVDBK_F64).  Forward conv:
+ * a = W^T per offset ([K][Cout][Cin]); dgrad: map = transposed map, in = grad_out,
+ * a = W ([K][Cin][Cout]).  Each output row is produced by one wavefront in a fixed
+ * (k ascending, channel chunk ascending) order, so a row's value does not depend on
+ * which other rows share the launch.  flags: reserved, pass 0.
+ * ------------------------------------------------------------------------- */
+int fvdbk_conv_implicit_gemm(fvdbk_dtype dtype, const void* in, int64_t n_in, int32_t c_in, const void* a,
+                             const int32_t* map, int64_t n_rows, int32_t K, int32_t M, void* out, uint32_t flags,
+                             hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Weight gradient (replaces dispatchSparseConvolutionImplicitGEMMGrad{,Sorted},
+ * backend/SparseConvolutionImplicitGEMMGrad.h, and the kernel-gradient half of
+ * dispatchSparseConvolutionKernelMapGrad, backend/SparseConvolutionKernelMap.h:23-30).
+ *
+ *   gw[k][c][o] = sum over pairs p of offset k:  in[pairs[p][ci]][c] * gout[pairs[p][co]][o]
+ *
+ * with (ci, co) = (0, 1), or (1, 0) when transposed.  Pairs are the neighbour map
+ * (ordered by k).  Split-K: the pairs of each offset are cut into chunks of
+ * `chunk` pairs (chunk_start[k] = first chunk of offset k, chunk_start[K] = total
+ * chunks, a device int32 array); each chunk's fp32 partial product goes to
+ * workspace [total_chunks][c_in][c_out] and a second pass adds the partials of each
+ * offset with Neumaier-compensated fp32 summation (fp64 plain sums for FVDBK_F64),
+ * in chunk order, so the result is deterministic.  k_offsets [K+1] (device int64)
+ * are the pair offsets of each k.  gw is fp32 ([K][c_in][c_out]) for the 16/32-bit
+ * dtypes and fp64 for FVDBK_F64.
+ * ------------------------------------------------------------------------- */
+size_t fvdbk_conv_wgrad_workspace_size(fvdbk_dtype dtype, int64_t total_chunks, int32_t c_in, int32_t c_out);
+
+int fvdbk_conv_wgrad(fvdbk_dtype dtype, const void* in, int32_t c_in, const void* gout, int32_t c_out,
+                     const int32_t* pairs, const int64_t* k_offsets, const int32_t* chunk_start, int32_t K,
+                     int32_t chunk, int64_t total_chunks, int32_t transposed, void* workspace,
+                     size_t workspace_bytes, void* gw, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FVDBK_H */

@@ -1,0 +1,327 @@
+"""GPU parity of the gfx950 path against the CPU oracle and the golden fixtures.
+
+Integer results (voxel order, target grids, kernel maps, transposed maps, neighbour
+pairs) must be bit-exact.  Floating-point tolerances, stated per test and measured as
+max|gpu - ref| / max|ref| against the float64 oracle:
+
+* fp64 on the GPU: 1e-12
+* fp32:            1e-5   (the north-star bound)
+* fp16:            2e-3
+* bf16:            1e-2   (the north-star bound)
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+REL_TOL = {torch.float64: 1e-12, torch.float32: 1e-5, torch.float16: 2e-3, torch.bfloat16: 1e-2}
+
+
+def _fvdb():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import fvdb
+
+    return fvdb
+
+
+def rel_err(a: torch.Tensor, ref: torch.Tensor) -> float:
+    a = a.detach().double().cpu()
+    ref = ref.detach().double().cpu()
+    den = ref.abs().max().item()
+    if den == 0:
+        return a.abs().max().item() if a.numel() else 0.0
+    return (a - ref).abs().max().item() / den
+
+
+def golden(name):
+    z = np.load(os.path.join(GOLDEN, name))
+    return {k: z[k] for k in z.files}
+
+
+GOLDEN_FILES = sorted(f for f in os.listdir(GOLDEN) if f.endswith(".npz"))
+
+
+def grids_from_fixture(fvdb, z):
+    offs = z["src_joffsets"]
+    raw = [torch.from_numpy(z["src_ijk_raw"][offs[b]:offs[b + 1]]).cuda() for b in range(len(offs) - 1)]
+    src = fvdb.GridBatch.from_ijk(fvdb.JaggedTensor(raw))
+    return src
+
+
+@pytest.mark.parametrize("fname", GOLDEN_FILES)
+def test_grid_order_and_kmap_bit_exact(fname):
+    fvdb = _fvdb()
+    z = golden(fname)
+    print(f"[case] {fname}", flush=True)
+    src = grids_from_fixture(fvdb, z)
+    assert np.array_equal(src.ijk.jdata.cpu().numpy(), z["src_ijk"])
+    assert np.array_equal(src.joffsets.cpu().numpy(), z["src_voxel_joffsets"])
+    ks = tuple(int(v) for v in z["kernel_size"])
+    st = tuple(int(v) for v in z["stride"])
+    pack, tgt = src.sparse_conv_kernel_map(ks, st)
+    assert np.array_equal(tgt.ijk.jdata.cpu().numpy(), z["tgt_ijk"])
+    assert np.array_equal(tgt.joffsets.cpu().numpy(), z["tgt_joffsets"])
+    pack.build_implicit_gemm(training=True)
+    km = pack.out_in_map
+    n_t = tgt.total_voxels
+    assert km.shape[0] == (n_t + 127) // 128 * 128
+    assert np.array_equal(km[:n_t].cpu().numpy(), z["kmap"])
+    assert bool((km[n_t:] == -1).all())
+    tm = pack.out_in_map_bwd
+    ref_t = oracle.transpose_map(z["kmap"], src.total_voxels)
+    assert np.array_equal(tm[:src.total_voxels].cpu().numpy(), ref_t)
+    pack.build_gather_scatter()
+    nb_ref, sz_ref = oracle.neighbor_map(z["kmap"])
+    assert np.array_equal(pack.neighborhood_map.cpu().numpy(), nb_ref)
+    assert np.array_equal(pack.neighborhood_sizes.cpu().numpy(), sz_ref)
+
+
+@pytest.mark.parametrize("fname", GOLDEN_FILES)
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.float16, torch.bfloat16])
+def test_forward_vs_golden(fname, dtype):
+    fvdb = _fvdb()
+    z = golden(fname)
+    print(f"[case] {fname} {dtype}", flush=True)
+    src = grids_from_fixture(fvdb, z)
+    ks = tuple(int(v) for v in z["kernel_size"])
+    st = tuple(int(v) for v in z["stride"])
+    pack, tgt = src.sparse_conv_kernel_map(ks, st)
+    pack.build_implicit_gemm()
+    x = torch.from_numpy(z["feat"]).to("cuda", dtype)
+    w = torch.from_numpy(z["weight"]).to("cuda", dtype)
+    out = pack.sparse_conv_3d(x, w, fvdb.ConvPackBackend.IGEMM).jdata
+    # reference from the rounded inputs, so only the contraction's error is measured
+    ref = oracle.sparse_conv(x.double().cpu(), w.double().cpu(), z["kmap"], z["kmap"].shape[0], st)
+    err = rel_err(out, ref)
+    print(f"  rel err {err:.3e}", flush=True)
+    assert err <= REL_TOL[dtype]
+    if dtype == torch.float64:
+        assert rel_err(out, torch.from_numpy(z["out"])) <= 1e-12
+
+
+def test_mfma_fragment_layout_exact():
+    """Small-integer data: every product and partial sum is exact in fp32, so bf16/f16/f32
+    results must equal the fp64 result exactly -- any operand or output lane mix-up fails."""
+    fvdb = _fvdb()
+    g = torch.Generator().manual_seed(0)
+    for cin, cout in [(32, 16), (64, 64), (8, 24), (3, 5), (96, 130)]:
+        n = 300
+        grid = fvdb.GridBatch.from_ijk(torch.randint(0, 9, (n, 3), generator=g, dtype=torch.int32).cuda())
+        pack, _ = grid.sparse_conv_kernel_map(3, 1)
+        pack.build_implicit_gemm()
+        N = grid.total_voxels
+        x = torch.randint(-3, 4, (N, cin), generator=g).double()
+        w = torch.randint(-2, 3, (cout, cin, 3, 3, 3), generator=g).double()
+        ref = oracle.sparse_conv(x, w, pack.out_in_map[:N].cpu().numpy(), N, 1)
+        for dt in (torch.float32, torch.float16, torch.bfloat16):
+            print(f"[case] cin={cin} cout={cout} {dt}", flush=True)
+            out = pack.sparse_conv_3d(x.to("cuda", dt), w.to("cuda", dt), fvdb.ConvPackBackend.IGEMM).jdata
+            if dt == torch.float32 or ref.abs().max() < 256:
+                assert torch.equal(out.double().cpu(), ref), f"max diff {(out.double().cpu() - ref).abs().max()}"
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16, torch.float16])
+@pytest.mark.parametrize("transposed", [False, True])
+def test_backward_vs_oracle(dtype, transposed):
+    fvdb = _fvdb()
+    z = golden("sphere_k3s2_c4x4.npz") if not transposed else golden("sphere_k3s1_c8.npz")
+    print(f"[case] backward {dtype} transposed={transposed}", flush=True)
+    src = grids_from_fixture(fvdb, z)
+    ks = tuple(int(v) for v in z["kernel_size"])
+    st = tuple(int(v) for v in z["stride"])
+    pack, tgt = src.sparse_conv_kernel_map(ks, st)
+    pack.build_implicit_gemm(training=True)
+    torch.manual_seed(3)
+    cin, cout = 8, 12
+    n_in = tgt.total_voxels if transposed else src.total_voxels
+    n_out = src.total_voxels if transposed else tgt.total_voxels
+    x = torch.randn(n_in, cin, dtype=torch.float64)
+    w = (torch.randn(cin, cout, *ks) if transposed else torch.randn(cout, cin, *ks)).double() * 0.2
+    R = torch.randn(n_out, cout, dtype=torch.float64)
+    xd = x.to("cuda", dtype).requires_grad_(True)
+    wd = w.to("cuda", dtype).requires_grad_(True)
+    if transposed:
+        out = pack.sparse_transpose_conv_3d(xd, wd, fvdb.ConvPackBackend.IGEMM).jdata
+    else:
+        out = pack.sparse_conv_3d(xd, wd, fvdb.ConvPackBackend.IGEMM).jdata
+    (out * R.to("cuda", dtype)).sum().backward()
+    xr, wr = xd.detach().double().cpu(), wd.detach().double().cpu()
+    kmap = z["kmap"]
+    ref_out = oracle.sparse_conv(xr, wr, kmap, n_out, st, transposed=transposed)
+    gi, gw = oracle.sparse_conv_backward(xr, R.to(dtype).double(), wr, kmap, ks, transposed=transposed)
+    tol = REL_TOL[dtype]
+    e_out, e_gi, e_gw = rel_err(out, ref_out), rel_err(xd.grad, gi), rel_err(wd.grad, gw)
+    print(f"  out {e_out:.2e} dgrad {e_gi:.2e} wgrad {e_gw:.2e}", flush=True)
+    assert e_out <= tol and e_gi <= tol
+    # wgrad of a 16-bit dtype is rounded once more to the dtype at the end
+    assert e_gw <= (tol if dtype in (torch.float64, torch.float32) else 2 * tol)
+
+
+def dense_box_grid(fvdb, dims):
+    return fvdb.GridBatch.from_dense(1, dims, device="cuda")
+
+
+@pytest.mark.parametrize("ks", [2, 3, 4, 5])
+@pytest.mark.parametrize("stride", [1, 2])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.bfloat16])
+def test_dense_conv3d_equivalence(ks, stride, dtype):
+    """``tests/unit/test_conv.py:361-461`` on a 12^3 dense grid (4 -> 8 channels)."""
+    fvdb = _fvdb()
+    print(f"[case] dense ks={ks} stride={stride} {dtype}", flush=True)
+    torch.manual_seed(0)
+    grid = dense_box_grid(fvdb, (12, 12, 12))
+    x = torch.randn(grid.total_voxels, 4, dtype=torch.float64)
+    w = torch.empty(8, 4, ks, ks, ks, dtype=torch.float64)
+    torch.nn.init.kaiming_uniform_(w, a=math.sqrt(5))
+    pack, tgt = grid.sparse_conv_kernel_map(ks, stride)
+    pack.build_gather_scatter()
+    out = pack.sparse_conv_3d(x.to("cuda", dtype), w.to("cuda", dtype), fvdb.ConvPackBackend.GATHER_SCATTER)
+    ijk = grid.ijk.jdata.long().cpu()
+    dense = torch.zeros(4, 12, 12, 12, dtype=torch.float64)
+    xr = x.to(dtype).double()
+    dense[:, ijk[:, 2], ijk[:, 1], ijk[:, 0]] = xr.t()
+    ref = torch.nn.functional.conv3d(dense[None], w.to(dtype).double(), padding=(ks - 1) // 2, stride=stride)[0]
+    t = tgt.ijk.jdata.long().cpu()
+    keep = (t[:, 0] < ref.shape[3]) & (t[:, 1] < ref.shape[2]) & (t[:, 2] < ref.shape[1]) & (t >= 0).all(1)
+    ref_rows = ref[:, t[keep, 2], t[keep, 1], t[keep, 0]].t()
+    err = rel_err(out.jdata.cpu()[keep], ref_rows)
+    print(f"  rel err {err:.2e}", flush=True)
+    assert err <= REL_TOL[dtype]
+
+
+@pytest.mark.parametrize("ks", [2, 3, 4])
+def test_transposed_vs_conv_transpose3d(ks):
+    fvdb = _fvdb()
+    torch.manual_seed(1)
+    grid = dense_box_grid(fvdb, (10, 10, 10))
+    pack, tgt = grid.sparse_conv_kernel_map(ks, 1)
+    pack.build_implicit_gemm(training=True)
+    x = torch.randn(tgt.total_voxels, 4, dtype=torch.float64)
+    w = torch.randn(4, 6, ks, ks, ks, dtype=torch.float64)
+    out = pack.sparse_transpose_conv_3d(x.float().cuda(), w.float().cuda(), fvdb.ConvPackBackend.IGEMM)
+    ijk = tgt.ijk.jdata.long().cpu()
+    dense = torch.zeros(4, 10, 10, 10, dtype=torch.float64)
+    dense[:, ijk[:, 2], ijk[:, 1], ijk[:, 0]] = x.float().double().t()
+    ref = torch.nn.functional.conv_transpose3d(dense[None], w.float().double(), padding=(ks - 1) // 2)[0]
+    ref = ref[:, :10, :10, :10]
+    s = grid.ijk.jdata.long().cpu()
+    ref_rows = ref[:, s[:, 2], s[:, 1], s[:, 0]].t()
+    assert rel_err(out.jdata.cpu(), ref_rows) <= 1e-5
+
+
+def test_error_inputs_raise_value_error():
+    """``tests/unit/test_conv.py:677-716``."""
+    fvdb = _fvdb()
+    torch.manual_seed(0)
+    grid = fvdb.GridBatch.from_points(fvdb.JaggedTensor(torch.randn(1000, 3, device="cuda")), voxel_sizes=0.025,
+                                      origins=[0.0, 0.0, 0.0])
+    n = int(grid.num_voxels[0].item())
+    for dtype in (torch.float32, torch.bfloat16):
+        for transposed in (False, True):
+            def run(feats, kernels):
+                kmap, _ = grid.sparse_conv_kernel_map(kernels.shape[-1], 1)
+                kmap.build_gather_scatter()
+                f = kmap.sparse_transpose_conv_3d if transposed else kmap.sparse_conv_3d
+                return f(feats, kernels, fvdb.ConvPackBackend.GATHER_SCATTER)
+
+            with pytest.raises(ValueError):
+                run(torch.randn(n, device="cuda", dtype=dtype), torch.rand(8, 0, 5, 5, 5, device="cuda", dtype=dtype))
+            with pytest.raises(ValueError):
+                run(torch.randn(n, 5, device="cuda", dtype=dtype), torch.rand(8, 4, 5, 5, 5, device="cuda", dtype=dtype))
+            with pytest.raises(ValueError):
+                run(torch.randn(n + 7, 4, device="cuda", dtype=dtype),
+                    torch.rand(8, 4, 5, 5, 5, device="cuda", dtype=dtype))
+
+
+def test_conv_empty_grid():
+    """``tests/unit/test_basic_ops.py:1694-1711``: an empty grid gives a [0, C] output."""
+    fvdb = _fvdb()
+    grid = fvdb.GridBatch.from_zero_voxels(device="cuda")
+    pack, tgt = grid.sparse_conv_kernel_map(3, 1)
+    pack.build_implicit_gemm()
+    out = pack.sparse_conv_3d(torch.empty(0, 4, device="cuda"), torch.randn(8, 4, 3, 3, 3, device="cuda"),
+                              fvdb.ConvPackBackend.IGEMM)
+    assert tuple(out.jdata.shape) == (0, 8)
+
+
+def test_batch_invariance_bit_exact():
+    """A grid's output rows are the same alone and inside a batch of other grids."""
+    fvdb = _fvdb()
+    z = golden("batch_k3s1_c4x6.npz")
+    g = torch.Generator().manual_seed(5)
+    offs = z["src_joffsets"]
+    raw = [torch.from_numpy(z["src_ijk_raw"][offs[b]:offs[b + 1]]).cuda() for b in range(len(offs) - 1)]
+    batch = fvdb.GridBatch.from_ijk(fvdb.JaggedTensor(raw))
+    w = torch.randn(64, 32, 3, 3, 3, generator=g).cuda()
+    for dt in (torch.bfloat16, torch.float32):
+        x = torch.randn(batch.total_voxels, 32, generator=g).to("cuda", dt)
+        pb, _ = batch.sparse_conv_kernel_map(3, 1)
+        pb.build_implicit_gemm()
+        yb = pb.sparse_conv_3d(x, w.to(dt), fvdb.ConvPackBackend.IGEMM).jdata
+        jo = batch.joffsets.cpu().tolist()
+        for b in range(batch.grid_count):
+            single = fvdb.GridBatch.from_ijk(raw[b])
+            ps, _ = single.sparse_conv_kernel_map(3, 1)
+            ps.build_implicit_gemm()
+            ys = ps.sparse_conv_3d(x[jo[b]:jo[b + 1]], w.to(dt), fvdb.ConvPackBackend.IGEMM).jdata
+            assert torch.equal(ys, yb[jo[b]:jo[b + 1]])
+
+
+def test_finite_difference_fp64():
+    """Central differences in fp64 on the GPU against the hand-written dgrad/wgrad."""
+    fvdb = _fvdb()
+    torch.manual_seed(7)
+    rng = np.random.default_rng(7)
+    ijk = torch.from_numpy(rng.integers(0, 14, size=(1500, 3)).astype(np.int32)).cuda()
+    grid = fvdb.GridBatch.from_ijk(ijk)
+    pack, _ = grid.sparse_conv_kernel_map(3, 1)
+    pack.build_implicit_gemm(training=True)
+    N = grid.total_voxels
+    x = torch.randn(N, 8, dtype=torch.float64, device="cuda", requires_grad=True)
+    w = (0.2 * torch.randn(8, 8, 3, 3, 3, dtype=torch.float64, device="cuda")).requires_grad_(True)
+    R = torch.randn(N, 8, dtype=torch.float64, device="cuda")
+
+    def loss(xx, ww):
+        return (pack.sparse_conv_3d(xx, ww, fvdb.ConvPackBackend.IGEMM).jdata * R).sum()
+
+    loss(x, w).backward()
+    h = 1e-6
+    for idx in [(0, 0), (5, 3), (N - 1, 7), (N // 2, 2)]:
+        xp = x.detach().clone(); xp[idx] += h
+        xm = x.detach().clone(); xm[idx] -= h
+        fd = (loss(xp, w.detach()) - loss(xm, w.detach())).item() / (2 * h)
+        assert abs(fd - x.grad[idx].item()) <= 1e-6 * max(1.0, abs(fd))
+    for idx in [(0, 0, 0, 0, 0), (3, 5, 1, 2, 0), (7, 7, 2, 2, 2), (1, 6, 1, 1, 1)]:
+        wp = w.detach().clone(); wp[idx] += h
+        wm = w.detach().clone(); wm[idx] -= h
+        fd = (loss(x.detach(), wp) - loss(x.detach(), wm)).item() / (2 * h)
+        assert abs(fd - w.grad[idx].item()) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_module_and_kmap_cache():
+    fvdb = _fvdb()
+    torch.manual_seed(0)
+    grid = dense_box_grid(fvdb, (8, 8, 8))
+    x = fvdb.nn.VDBTensor(grid, grid.jagged_like(torch.randn(grid.total_voxels, 16, device="cuda")))
+    c1 = fvdb.nn.SparseConv3d(16, 16, 3).cuda()
+    c2 = fvdb.nn.SparseConv3d(16, 32, 3).cuda()
+    y = c1(x)
+    assert y.kmap is not None and y.grid.is_same(grid)
+    z = c2(y)
+    assert z.kmap is y.kmap
+    down = fvdb.nn.SparseConv3d(32, 32, 2, stride=2).cuda()
+    d = down(z)
+    assert d.kmap is None and d.grid.total_voxels == 64
+    up = fvdb.nn.SparseConv3d(32, 16, 2, stride=2, transposed=True).cuda()
+    u = up(d, out_grid=grid)
+    assert u.grid.is_same(grid) and tuple(u.data.jdata.shape) == (grid.total_voxels, 16)
+    (u.data.jdata.sum()).backward()
+    assert c1.weight.grad is not None and up.weight.grad is not None
