@@ -60,7 +60,7 @@ def run_step(fvdb, grid, x, wa, wb):
     IG = fvdb.ConvPackBackend.IGEMM
     pa, _ = grid.sparse_conv_kernel_map(3, 1)
     pa.build_implicit_gemm()
-    ya = pa.sparse_conv_3d(x, wa, IG)
+    ya = pa.sparse_conv_3d(grid.jagged_like(x), wa, IG)
     pb, coarse = grid.sparse_conv_kernel_map(2, 2)
     pb.build_implicit_gemm()
     yb = pb.sparse_conv_3d(ya, wb, IG)
@@ -226,7 +226,8 @@ def main() -> int:
     def conv_only(dt):
         xa, w1, w2 = x.to(dt), wa.to(dt), wb.to(dt)
         IG = fvdb.ConvPackBackend.IGEMM
-        return lambda: pb.sparse_conv_3d(pa.sparse_conv_3d(xa, w1, IG), w2, IG)
+        xj = grid.jagged_like(xa)
+        return lambda: pb.sparse_conv_3d(pa.sparse_conv_3d(xj, w1, IG), w2, IG)
 
     log("[bench] timing conv-only")
     tc = {name: time_steps(torch, conv_only(dt), args.steps, args.warmup)

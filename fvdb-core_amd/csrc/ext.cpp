@@ -89,6 +89,10 @@ std::vector<at::Tensor> build_grid(at::Tensor ijk, at::Tensor jidx, int64_t n_gr
     if (has_jidx) {
         TORCH_CHECK_VALUE(jidx.numel() == ijk.size(0), "jidx must have one entry per voxel");
         jidx = jidx.to(at::kInt).contiguous();
+        if (jidx.numel() > 0) {
+            TORCH_CHECK_VALUE(jidx.max().item<int32_t>() < n_grids, "grid index out of range for ", n_grids,
+                              " grids");
+        }
     }
     c10::hip::HIPGuard guard(ijk.device().index());
     hipStream_t s = stream_of(ijk);

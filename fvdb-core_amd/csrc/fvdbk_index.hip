@@ -517,7 +517,8 @@ int fvdbk_conv_grid_candidates(const int32_t* ijk, const int32_t* jidx, int64_t 
 
 int fvdbk_kmap_build(const fvdbk_grid* rows, const fvdbk_grid* probe, const int32_t ks[3], const int32_t stride[3],
                      int32_t inverse, int32_t* map, int64_t n_rows_padded, hipStream_t stream) {
-    FVDBK_REQUIRE(rows && probe && map, FVDBK_EINVAL, "null argument");
+    FVDBK_REQUIRE(rows && probe, FVDBK_EINVAL, "null grid");
+    FVDBK_REQUIRE(map || n_rows_padded == 0, FVDBK_EINVAL, "null kernel map");
     for (int d = 0; d < 3; ++d) {
         FVDBK_REQUIRE(ks[d] > 0, FVDBK_EINVAL, "Expect kernel size to be larger than {0,0,0}.");
         FVDBK_REQUIRE(stride[d] > 0, FVDBK_EINVAL, "Expect stride to be larger than 0.");

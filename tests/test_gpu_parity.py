@@ -97,7 +97,7 @@ def test_forward_vs_golden(fname, dtype):
     pack.build_implicit_gemm()
     x = torch.from_numpy(z["feat"]).to("cuda", dtype)
     w = torch.from_numpy(z["weight"]).to("cuda", dtype)
-    out = pack.sparse_conv_3d(x, w, fvdb.ConvPackBackend.IGEMM).jdata
+    out = pack.sparse_conv_3d(src.jagged_like(x), w, fvdb.ConvPackBackend.IGEMM).jdata
     # reference from the rounded inputs, so only the contraction's error is measured
     ref = oracle.sparse_conv(x.double().cpu(), w.double().cpu(), z["kmap"], z["kmap"].shape[0], st)
     err = rel_err(out, ref)
@@ -149,9 +149,9 @@ def test_backward_vs_oracle(dtype, transposed):
     xd = x.to("cuda", dtype).requires_grad_(True)
     wd = w.to("cuda", dtype).requires_grad_(True)
     if transposed:
-        out = pack.sparse_transpose_conv_3d(xd, wd, fvdb.ConvPackBackend.IGEMM).jdata
+        out = pack.sparse_transpose_conv_3d(tgt.jagged_like(xd), wd, fvdb.ConvPackBackend.IGEMM).jdata
     else:
-        out = pack.sparse_conv_3d(xd, wd, fvdb.ConvPackBackend.IGEMM).jdata
+        out = pack.sparse_conv_3d(src.jagged_like(xd), wd, fvdb.ConvPackBackend.IGEMM).jdata
     (out * R.to("cuda", dtype)).sum().backward()
     xr, wr = xd.detach().double().cpu(), wd.detach().double().cpu()
     kmap = z["kmap"]
@@ -265,7 +265,7 @@ def test_batch_invariance_bit_exact():
         x = torch.randn(batch.total_voxels, 32, generator=g).to("cuda", dt)
         pb, _ = batch.sparse_conv_kernel_map(3, 1)
         pb.build_implicit_gemm()
-        yb = pb.sparse_conv_3d(x, w.to(dt), fvdb.ConvPackBackend.IGEMM).jdata
+        yb = pb.sparse_conv_3d(batch.jagged_like(x), w.to(dt), fvdb.ConvPackBackend.IGEMM).jdata
         jo = batch.joffsets.cpu().tolist()
         for b in range(batch.grid_count):
             single = fvdb.GridBatch.from_ijk(raw[b])
