@@ -138,7 +138,7 @@ std::vector<at::Tensor> build_grid(at::Tensor ijk, at::Tensor jidx, int64_t n_gr
     at::Tensor gcounts = at::empty({n_grids}, o32);
     check_rc(fvdbk_grid_finalize(reinterpret_cast<const uint64_t*>(ukeys.data_ptr<int64_t>()),
                                  reinterpret_cast<const uint64_t*>(masks.data_ptr<int64_t>()), start.data_ptr<int64_t>(),
-                                 L, wbase.data_ptr<int32_t>(), ijk_out.data_ptr<int32_t>(), jidx_out.data_ptr<int32_t>(),
+                                 L, nv, wbase.data_ptr<int32_t>(), ijk_out.data_ptr<int32_t>(), jidx_out.data_ptr<int32_t>(),
                                  gcounts.data_ptr<int32_t>(), (int32_t)n_grids, s),
              "grid_finalize");
     return {ijk_out, jidx_out, gcounts, ukeys, masks, wbase, hash};

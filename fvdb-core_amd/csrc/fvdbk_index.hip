@@ -83,18 +83,40 @@ __global__ void k_zero_u64(uint64_t* __restrict__ p, int64_t n) {
     if (i < n) p[i] = 0;
 }
 
-__global__ void k_leaf_masks(const int32_t* __restrict__ ijk, const int32_t* __restrict__ bidx, int64_t n,
-                             const fvdbk_hslot* __restrict__ h, int64_t cap, uint64_t* __restrict__ masks) {
+// Mask bits of every input voxel.  Inputs usually arrive in tree order (candidates
+// derived from an existing grid), so neighbouring lanes often hit the same mask word:
+// lanes OR their bits with the lanes before them that share the word (segmented
+// shuffle scan) and only the last lane of each run issues the atomic.
+__global__ __launch_bounds__(256) void k_leaf_masks(const int32_t* __restrict__ ijk, const int32_t* __restrict__ bidx,
+                                                    int64_t n, const fvdbk_hslot* __restrict__ h, int64_t cap,
+                                                    uint64_t* __restrict__ masks) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int32_t b = bidx ? bidx[i] : 0;
-    if (b < 0 || b >= kMaxGrids) return;
-    const int32_t x = ijk[3 * i], y = ijk[3 * i + 1], z = ijk[3 * i + 2];
-    if (x < kCoordMin || x > kCoordMax || y < kCoordMin || y > kCoordMax || z < kCoordMin || z > kCoordMax) return;
-    const int32_t leaf = hash_find(h, cap, leaf_key(b, x, y, z));
-    if (leaf < 0) return;  // cannot happen for keys produced by k_leaf_keys
-    const int bit = ((x & 7) << 6) | ((y & 7) << 3) | (z & 7);
-    atomicOr((unsigned long long*)&masks[(int64_t)leaf * 8 + (bit >> 6)], 1ull << (bit & 63));
+    const int lane = threadIdx.x & 63;
+    int64_t word = -1;
+    uint64_t bits = 0;
+    if (i < n) {
+        const int32_t b = bidx ? bidx[i] : 0;
+        const int32_t x = ijk[3 * i], y = ijk[3 * i + 1], z = ijk[3 * i + 2];
+        if (b >= 0 && b < kMaxGrids && x >= kCoordMin && x <= kCoordMax && y >= kCoordMin && y <= kCoordMax &&
+            z >= kCoordMin && z <= kCoordMax) {
+            const int32_t leaf = hash_find(h, cap, leaf_key(b, x, y, z));
+            if (leaf >= 0) {
+                const int bit = ((x & 7) << 6) | ((y & 7) << 3) | (z & 7);
+                word = (int64_t)leaf * 8 + (bit >> 6);
+                bits = 1ull << (bit & 63);
+            }
+        }
+    }
+    // inclusive segmented OR over runs of equal `word` (runs are contiguous lanes)
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t ob = __shfl_up(bits, d);
+        const int64_t ow = __shfl_up(word, d);
+        if (lane >= d && ow == word) bits |= ob;
+    }
+    const int64_t next = __shfl_down(word, 1);
+    const bool tail = lane == 63 || next != word;
+    if (word >= 0 && tail) atomicOr((unsigned long long*)&masks[word], bits);
 }
 
 __global__ void k_leaf_counts(const uint64_t* __restrict__ masks, int64_t n_leaves, int32_t* __restrict__ counts) {
@@ -116,8 +138,7 @@ __global__ __launch_bounds__(256) void k_finalize(const uint64_t* __restrict__ k
                                                   const uint64_t* __restrict__ masks,
                                                   const int64_t* __restrict__ leaf_start, int64_t n_leaves,
                                                   int32_t* __restrict__ word_base, int32_t* __restrict__ ijk_out,
-                                                  int32_t* __restrict__ jidx_out, int32_t* __restrict__ grid_counts,
-                                                  int32_t n_grids) {
+                                                  int32_t* __restrict__ jidx_out) {
     const int lane = threadIdx.x & 63;
     const int64_t l = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (l >= n_leaves) return;
@@ -137,7 +158,27 @@ __global__ __launch_bounds__(256) void k_finalize(const uint64_t* __restrict__ k
         }
         base += __popcll(m);
     }
-    if (grid_counts && lane == 0 && b < n_grids) atomicAdd(&grid_counts[b], (int32_t)(base - leaf_start[l]));
+}
+
+// Voxels per grid: leaves are sorted by grid (the grid index is the key's top field),
+// so grid b owns the leaves between the first keys >= b<<51 and >= (b+1)<<51.
+__global__ void k_grid_counts(const uint64_t* __restrict__ keys, const int64_t* __restrict__ leaf_start,
+                              int64_t n_leaves, int64_t n_voxels, int32_t n_grids, int32_t* __restrict__ counts) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n_grids) return;
+    auto lower = [&](uint64_t key) {
+        int64_t lo = 0, hi = n_leaves;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (keys[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    const int64_t l0 = lower((uint64_t)b << 51), l1 = lower((uint64_t)(b + 1) << 51);
+    const int64_t v0 = l0 < n_leaves ? leaf_start[l0] : n_voxels;
+    const int64_t v1 = l1 < n_leaves ? leaf_start[l1] : n_voxels;
+    counts[b] = (int32_t)(v1 - v0);
 }
 
 // ---------------------------------------------------------------------------------
@@ -236,94 +277,157 @@ struct ProbeGeom {
     int ext[3];   // taps per coordinate axis (x: ks[2], y: ks[1], z: ks[0])
     int mul[3];
     int div[3];
+    int shift[3]; // log2(div) when div is a power of two, else -1
     int off0[3];
     int sgn;
+    int n_blocks; // workgroups holding a row leaf (the launch may be padded to a multiple of 8)
 };
 
-constexpr int KMAP_THREADS = 256;
-constexpr int KMAP_MAX_NB = 216;  // staged neighbourhood: up to 6x6x6 probe leaves
+// Workgroup -> work-item order that gives each of the 8 XCDs (which take workgroups
+// round-robin) one contiguous range of the work, so spatially adjacent tiles share an
+// L2.  Returns -1 for padding workgroups.
+__device__ inline int64_t xcd_swizzle(int64_t bid, int64_t n) {
+    const int64_t per = (n + 7) / 8;
+    const int64_t t = (bid % 8) * per + bid / 8;
+    return t < n ? t : -1;
+}
 
-// One workgroup per row-grid leaf.  The probe-grid leaves that the leaf's taps can
-// reach form a small box; their masks and word bases are staged in LDS once, then
-// every (row voxel, tap) pair of the leaf is resolved from LDS and the map block of
-// the leaf (contiguous rows) is written with consecutive threads on consecutive
-// int32 entries.  Boxes larger than KMAP_MAX_NB leaves fall back to a hash probe per
-// pair.
-__global__ __launch_bounds__(KMAP_THREADS) void k_kmap(fvdbk_grid rows, fvdbk_grid probe, ProbeGeom g,
-                                                       int32_t* __restrict__ map) {
-    __shared__ uint64_t s_mask[KMAP_MAX_NB * 8];
-    __shared__ int32_t s_base[KMAP_MAX_NB * 8];
-    __shared__ int32_t s_ijk[512 * 3];
+constexpr int KMAP_WAVES = 4;          // row leaves per workgroup (one wavefront each)
+constexpr int KMAP_STAGE_BYTES = 16384; // LDS per wavefront for the staged neighbourhood
 
-    const int64_t L = blockIdx.x;
-    const int tid = threadIdx.x;
-    int32_t b, o[3];
-    decode_leaf_key(rows.leaf_keys[L], b, o[0], o[1], o[2]);
-
-    const int64_t vbase = rows.leaf_word_base[L * 8];
-    const int cnt = (int)(rows.leaf_word_base[L * 8 + 7] - vbase) + __popcll(rows.leaf_masks[L * 8 + 7]);
-
-    int lo[3], ext[3];
-    for (int a = 0; a < 3; ++a) {
-        const int span = g.sgn * (g.ext[a] - 1);
-        const int off_lo = g.off0[a] + (span < 0 ? span : 0);
-        const int off_hi = g.off0[a] + (span > 0 ? span : 0);
-        const int q_lo = floordiv(o[a] * g.mul[a] + off_lo, g.div[a]);
-        const int q_hi = floordiv((o[a] + 7) * g.mul[a] + off_hi, g.div[a]);
-        lo[a] = q_lo >> 3;
-        ext[a] = (q_hi >> 3) - lo[a] + 1;
+// One wavefront per row-grid leaf, 4 leaves per workgroup.  The probe-grid leaves
+// that the leaf's taps can reach form a small box (nb_cap leaves at most, fixed per
+// launch); each lane stages one of them (hash lookup, then its 8 mask words and word
+// bases) into the wavefront's LDS slice.  The leaf's active voxels are listed with a
+// ballot compaction of its mask (bit position per voxel rank, no ijk reads), and the
+// leaf's block of the map (cnt*K consecutive int32) is written by consecutive lanes.
+// With nb_cap == 0 (box too large to stage) every pair does a hash probe instead.
+__global__ __launch_bounds__(256) void k_kmap(fvdbk_grid rows, fvdbk_grid probe, ProbeGeom g, int nb_cap,
+                                              int32_t* __restrict__ map) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char kmap_smem[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t blk = xcd_swizzle(blockIdx.x, g.n_blocks);
+    const int64_t L = blk < 0 ? rows.n_leaves : blk * KMAP_WAVES + wave;
+    // LDS slice of this wavefront: masks [nb_cap][8] u64, bases [nb_cap][8] i32, bits [512] u16
+    unsigned char* base_ptr = kmap_smem + (size_t)wave * (nb_cap * 96 + 1024);
+    uint64_t* s_mask = reinterpret_cast<uint64_t*>(base_ptr);
+    int32_t* s_base = reinterpret_cast<int32_t*>(base_ptr + nb_cap * 64);
+    uint16_t* s_bits = reinterpret_cast<uint16_t*>(base_ptr + nb_cap * 96);
+    // per-tap offsets (x, y, z), shared by the workgroup
+    int32_t* s_off = reinterpret_cast<int32_t*>(kmap_smem + (size_t)KMAP_WAVES * (nb_cap * 96 + 1024));
+    for (int k = threadIdx.x; k < g.K; k += 64 * KMAP_WAVES) {
+        const int tz = k % g.ext[2], ty = (k / g.ext[2]) % g.ext[1], tx = k / (g.ext[2] * g.ext[1]);
+        s_off[3 * k + 0] = g.off0[0] + g.sgn * tx;
+        s_off[3 * k + 1] = g.off0[1] + g.sgn * ty;
+        s_off[3 * k + 2] = g.off0[2] + g.sgn * tz;
     }
-    const int nn = ext[0] * ext[1] * ext[2];
-    const bool staged = nn <= KMAP_MAX_NB;
 
-    if (staged) {
-        for (int n = tid; n < nn; n += KMAP_THREADS) {
-            const int dz = n % ext[2], dy = (n / ext[2]) % ext[1], dx = n / (ext[2] * ext[1]);
-            const int32_t lx = (lo[0] + dx) * 8, ly = (lo[1] + dy) * 8, lz = (lo[2] + dz) * 8;
-            int32_t leaf = -1;
-            if (probe.n_leaves > 0 && lx >= kCoordMin && lx <= kCoordMax && ly >= kCoordMin && ly <= kCoordMax &&
-                lz >= kCoordMin && lz <= kCoordMax)
-                leaf = hash_find(probe.hash, probe.hash_cap, leaf_key(b, lx, ly, lz));
-            for (int w = 0; w < 8; ++w) {
-                s_mask[n * 8 + w] = leaf >= 0 ? probe.leaf_masks[(int64_t)leaf * 8 + w] : 0ull;
-                s_base[n * 8 + w] = leaf >= 0 ? probe.leaf_word_base[(int64_t)leaf * 8 + w] : 0;
+    const bool live = L < rows.n_leaves;
+    int32_t b = 0, o[3] = {0, 0, 0};
+    int lo[3] = {0, 0, 0}, ext[3] = {1, 1, 1};
+    int64_t vbase = 0;
+    int cnt = 0;
+    bool staged = false;
+    if (live) {
+        decode_leaf_key(rows.leaf_keys[L], b, o[0], o[1], o[2]);
+        vbase = rows.leaf_word_base[L * 8];
+        // ballot compaction of the leaf's active bits into rank order
+        int rank = 0;
+        for (int w = 0; w < 8; ++w) {
+            const uint64_t m = rows.leaf_masks[L * 8 + w];
+            if ((m >> lane) & 1ull) s_bits[rank + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)(w * 64 + lane);
+            rank += __popcll(m);
+        }
+        cnt = rank;
+        for (int a = 0; a < 3; ++a) {
+            const int span = g.sgn * (g.ext[a] - 1);
+            const int off_lo = g.off0[a] + (span < 0 ? span : 0);
+            const int off_hi = g.off0[a] + (span > 0 ? span : 0);
+            const int q_lo = floordiv(o[a] * g.mul[a] + off_lo, g.div[a]);
+            const int q_hi = floordiv((o[a] + 7) * g.mul[a] + off_hi, g.div[a]);
+            lo[a] = q_lo >> 3;
+            ext[a] = (q_hi >> 3) - lo[a] + 1;
+        }
+        const int nn = ext[0] * ext[1] * ext[2];
+        staged = nn <= nb_cap;
+        if (staged) {
+            for (int n = lane; n < nn; n += 64) {
+                const int dz = n % ext[2], dy = (n / ext[2]) % ext[1], dx = n / (ext[2] * ext[1]);
+                const int32_t lx = (lo[0] + dx) * 8, ly = (lo[1] + dy) * 8, lz = (lo[2] + dz) * 8;
+                int32_t leaf = -1;
+                if (probe.n_leaves > 0 && lx >= kCoordMin && lx <= kCoordMax && ly >= kCoordMin && ly <= kCoordMax &&
+                    lz >= kCoordMin && lz <= kCoordMax)
+                    leaf = hash_find(probe.hash, probe.hash_cap, leaf_key(b, lx, ly, lz));
+                uint64_t mw[8];
+                int32_t bw[8];
+#pragma unroll
+                for (int w = 0; w < 8; ++w) {
+                    mw[w] = leaf >= 0 ? probe.leaf_masks[(int64_t)leaf * 8 + w] : 0ull;
+                    bw[w] = leaf >= 0 ? probe.leaf_word_base[(int64_t)leaf * 8 + w] : 0;
+                }
+#pragma unroll
+                for (int w = 0; w < 8; ++w) {
+                    s_mask[n * 8 + w] = mw[w];
+                    s_base[n * 8 + w] = bw[w];
+                }
             }
         }
     }
-    for (int e = tid; e < 3 * cnt; e += KMAP_THREADS) s_ijk[e] = rows.ijk[vbase * 3 + e];
     __syncthreads();
+    if (!live) return;
 
     const int K = g.K;
     const int total = cnt * K;
     int32_t* __restrict__ out = map + vbase * K;
-    for (int idx = tid; idx < total; idx += KMAP_THREADS) {
-        const int r = idx / K;
-        const int k = idx - r * K;
-        const int tz = k % g.ext[2];
-        const int ty = (k / g.ext[2]) % g.ext[1];
-        const int tx = k / (g.ext[2] * g.ext[1]);
-        const int t[3] = {tx, ty, tz};
+    const int qstep = 64 / K, rstep = 64 % K;
+    int r = lane / K, k = lane - (lane / K) * K;
+    for (int idx = lane; idx < total; idx += 64) {
+        const int bit = s_bits[r];
+        const int c[3] = {o[0] + (bit >> 6), o[1] + ((bit >> 3) & 7), o[2] + (bit & 7)};
         int32_t q[3];
         bool ok = true;
+#pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const int num = s_ijk[r * 3 + a] * g.mul[a] + g.off0[a] + g.sgn * t[a];
-            if (num % g.div[a] != 0) ok = false;
-            q[a] = num / g.div[a];
+            const int num = c[a] * g.mul[a] + s_off[3 * k + a];
+            if (g.shift[a] >= 0) {
+                if (num & ((1 << g.shift[a]) - 1)) ok = false;
+                q[a] = num >> g.shift[a];
+            } else {
+                if (num % g.div[a] != 0) ok = false;
+                q[a] = num / g.div[a];
+            }
         }
         int32_t res = -1;
         if (ok) {
             if (staged) {
                 const int n = (((q[0] >> 3) - lo[0]) * ext[1] + ((q[1] >> 3) - lo[1])) * ext[2] + ((q[2] >> 3) - lo[2]);
-                const int bit = ((q[0] & 7) << 6) | ((q[1] & 7) << 3) | (q[2] & 7);
-                const uint64_t m = s_mask[n * 8 + (bit >> 6)];
-                if ((m >> (bit & 63)) & 1ull)
-                    res = s_base[n * 8 + (bit >> 6)] + __popcll(m & ((1ull << (bit & 63)) - 1ull));
+                const int qb = ((q[0] & 7) << 6) | ((q[1] & 7) << 3) | (q[2] & 7);
+                const uint64_t m = s_mask[n * 8 + (qb >> 6)];
+                if ((m >> (qb & 63)) & 1ull)
+                    res = s_base[n * 8 + (qb >> 6)] + __popcll(m & ((1ull << (qb & 63)) - 1ull));
             } else if (probe.n_leaves > 0) {
                 res = voxel_index(probe, b, q[0], q[1], q[2]);
             }
         }
         out[idx] = res;
+        r += qstep;
+        k += rstep;
+        if (k >= K) {
+            k -= K;
+            ++r;
+        }
     }
+}
+
+// Upper bound on the probe leaves one row leaf can reach (per launch).
+static int kmap_box_bound(const ProbeGeom& g) {
+    int nb = 1;
+    for (int a = 0; a < 3; ++a) {
+        // probe coordinates span at most (7*mul + ext - 1)/div + 1 values
+        const int span_q = (7 * g.mul[a] + g.ext[a] - 1) / g.div[a] + 1;
+        nb *= (span_q + 7) / 8 + 1;
+    }
+    return nb;
 }
 
 __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
@@ -438,17 +542,18 @@ int fvdbk_grid_leaf_counts(const uint64_t* leaf_masks, int64_t n_leaves, int32_t
 }
 
 int fvdbk_grid_finalize(const uint64_t* leaf_keys, const uint64_t* leaf_masks, const int64_t* leaf_start,
-                        int64_t n_leaves, int32_t* leaf_word_base, int32_t* ijk_out, int32_t* jidx_out,
-                        int32_t* grid_counts, int32_t n_grids, hipStream_t stream) {
-    if (grid_counts && n_grids > 0) {
-        hipLaunchKernelGGL(k_zero_i32, dim3(grid_1d(n_grids, 256)), dim3(256), 0, stream, grid_counts,
-                           (int64_t)n_grids);
-        FVDBK_LAUNCH_CHECK("k_zero_i32");
+                        int64_t n_leaves, int64_t n_voxels, int32_t* leaf_word_base, int32_t* ijk_out,
+                        int32_t* jidx_out, int32_t* grid_counts, int32_t n_grids, hipStream_t stream) {
+    if (n_leaves > 0) {
+        hipLaunchKernelGGL(k_finalize, dim3(grid_1d(n_leaves, 4)), dim3(256), 0, stream, leaf_keys, leaf_masks,
+                           leaf_start, n_leaves, leaf_word_base, ijk_out, jidx_out);
+        FVDBK_LAUNCH_CHECK("k_finalize");
     }
-    if (n_leaves == 0) return FVDBK_OK;
-    hipLaunchKernelGGL(k_finalize, dim3(grid_1d(n_leaves, 4)), dim3(256), 0, stream, leaf_keys, leaf_masks,
-                       leaf_start, n_leaves, leaf_word_base, ijk_out, jidx_out, grid_counts, n_grids);
-    FVDBK_LAUNCH_CHECK("k_finalize");
+    if (grid_counts && n_grids > 0) {
+        hipLaunchKernelGGL(k_grid_counts, dim3(grid_1d(n_grids, 64)), dim3(64), 0, stream, leaf_keys, leaf_start,
+                           n_leaves, n_voxels, n_grids, grid_counts);
+        FVDBK_LAUNCH_CHECK("k_grid_counts");
+    }
     return FVDBK_OK;
 }
 
@@ -548,9 +653,20 @@ int fvdbk_kmap_build(const fvdbk_grid* rows, const fvdbk_grid* probe, const int3
         }
     }
     g.sgn = inverse ? -1 : 1;
+    for (int a = 0; a < 3; ++a) {
+        g.shift[a] = -1;
+        for (int sh = 0; sh < 31; ++sh)
+            if ((1 << sh) == g.div[a]) g.shift[a] = sh;
+    }
     if (rows->n_leaves > 0) {
-        hipLaunchKernelGGL(k_kmap, dim3((unsigned)rows->n_leaves), dim3(KMAP_THREADS), 0, stream, *rows, *probe, g,
-                           map);
+        int nb = kmap_box_bound(g);
+        if (nb * 96 + 1024 > KMAP_STAGE_BYTES) nb = 0;  // too large to stage: probe the hash per pair
+        const size_t lds = (size_t)KMAP_WAVES * (nb * 96 + 1024) + (size_t)K * 12;
+        FVDBK_REQUIRE(lds <= 65536, FVDBK_EINVAL, "kernel volume %lld too large for the kernel-map launch",
+                      (long long)K);
+        g.n_blocks = (int)((rows->n_leaves + KMAP_WAVES - 1) / KMAP_WAVES);
+        const unsigned blocks = (unsigned)((g.n_blocks + 7) / 8 * 8);
+        hipLaunchKernelGGL(k_kmap, dim3(blocks), dim3(64 * KMAP_WAVES), lds, stream, *rows, *probe, g, nb, map);
         FVDBK_LAUNCH_CHECK("k_kmap");
     }
     const int64_t tail = (n_rows_padded - rows->n_voxels) * K;

@@ -84,7 +84,7 @@ const char* fvdbk_version(void);
  * src/fvdb/detail/ops/BuildGridFromIjk.cu:27-84 and the per-voxel ijk/jidx export
  * of src/fvdb/detail/GridBatchImpl.cu).  A build is a short sequence because the
  * number of leaves is only known after de-duplicating keys:
- *   1. fvdbk_grid_leaf_keys     per input voxel: leaf sort key (UINT64_MAX = skip)
+ *   1. fvdbk_grid_leaf_keys     per input voxel: leaf sort key (INT64_MAX = skipped voxel)
  *   2. caller sorts + de-duplicates the keys (any device sort) -> n_leaves keys
  *   3. fvdbk_grid_hash_build    leaf hash over the unique keys
  *   4. fvdbk_grid_leaf_masks    512-bit masks (OR of the input voxels)
@@ -94,8 +94,9 @@ const char* fvdbk_version(void);
  * ------------------------------------------------------------------------- */
 
 /* batch_idx may be NULL (all voxels in grid 0); entries < 0 are skipped.
- * Coordinates must lie in [-2^20, 2^20); *bad_count (device int32) is incremented
- * per out-of-range voxel (caller zeroes it and checks it). */
+ * Coordinates must lie in [-2^19, 2^19) and grid indices below 1024; *bad_count
+ * (device int32) is incremented
+ * per out-of-range voxel (the caller zeroes it and checks it). */
 int fvdbk_grid_leaf_keys(const int32_t* ijk, const int32_t* batch_idx, int64_t n, uint64_t* keys,
                          int32_t* bad_count, hipStream_t stream);
 
@@ -109,11 +110,11 @@ int fvdbk_grid_leaf_masks(const int32_t* ijk, const int32_t* batch_idx, int64_t 
 int fvdbk_grid_leaf_counts(const uint64_t* leaf_masks, int64_t n_leaves, int32_t* counts, hipStream_t stream);
 
 /* Writes leaf_word_base [n_leaves][8], ijk_out [n_voxels][3], jidx_out [n_voxels]
- * (grid index per voxel; may be NULL) and grid_counts [n_grids] (voxels per grid,
- * cleared by the call; may be NULL). */
+ * (grid index per voxel; may be NULL) and grid_counts [n_grids] (voxels per grid; may
+ * be NULL).  leaf_start is the exclusive scan of the leaf counts, n_voxels their sum. */
 int fvdbk_grid_finalize(const uint64_t* leaf_keys, const uint64_t* leaf_masks, const int64_t* leaf_start,
-                        int64_t n_leaves, int32_t* leaf_word_base, int32_t* ijk_out, int32_t* jidx_out,
-                        int32_t* grid_counts, int32_t n_grids, hipStream_t stream);
+                        int64_t n_leaves, int64_t n_voxels, int32_t* leaf_word_base, int32_t* ijk_out,
+                        int32_t* jidx_out, int32_t* grid_counts, int32_t n_grids, hipStream_t stream);
 
 /* Linear index of query coordinates (-1 when inactive).  Replaces
  * src/fvdb/detail/ops/IjkToIndex.cu (GridBatch.ijk_to_index, cumulative indices). */
