@@ -182,6 +182,11 @@ def main() -> int:
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only loop the dominant kernel (for profiler runs)")
+    ap.add_argument("--workload", default="precision", choices=["precision", "single", "diff", "unet", "serve"],
+                    help="precision (default, the headline line); single / diff / unet / serve: the other "
+                         "workloads, one JSON line each")
+    ap.add_argument("--requests", type=int, default=120, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=0.0, help="serve: Poisson arrival rate (req/s); 0 = auto")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -194,6 +199,11 @@ def main() -> int:
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.workload != "precision":
+        rec = {"single": run_single, "diff": run_diff, "unet": run_unet, "serve": run_serve}[args.workload](
+            torch, fvdb, dev, args)
+        print(json.dumps(rec), flush=True)
+        return 0
     log("[bench] building workload")
     grid, vs, x, wa, wb = build_workload(torch, fvdb, dev)
     N = grid.total_voxels
@@ -307,6 +317,289 @@ def main() -> int:
     }
     print(json.dumps(rec), flush=True)
     return 0
+
+
+# ---------------------------------------------------------------------------------------------
+# other workloads (one JSON line each; the default line above is the headline metric)
+# ---------------------------------------------------------------------------------------------
+
+
+def _median_ms(torch, fn, iters=10, warmup=2):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(iters):
+        s = torch.cuda.Event(enable_timing=True)
+        e = torch.cuda.Event(enable_timing=True)
+        s.record()
+        fn()
+        e.record()
+        e.synchronize()
+        ts.append(s.elapsed_time(e))
+    return statistics.median(ts)
+
+
+def run_single(torch, fvdb, dev, args):
+    """One grid: 2M points on 8 spheres at 1/256, 3x3x3 32->32 fp32; kernel map + forward."""
+    import numpy as np
+
+    import oracle
+    from fvdb.utils.synthetic import sphere_points
+
+    pts = sphere_points(seed=0, n_points=2_000_000, device=dev)
+    grid = fvdb.GridBatch.from_points(pts, voxel_sizes=1 / 256)
+    N = grid.total_voxels
+    g = torch.Generator(device=dev).manual_seed(0)
+    x = torch.randn((N, 32), generator=g, device=dev)
+    w = torch.randn((32, 32, 3, 3, 3), generator=g, device=dev) * math.sqrt(2.0 / (32 * 27))
+    IG = fvdb.ConvPackBackend.IGEMM
+
+    def kmap():
+        p, _ = grid.sparse_conv_kernel_map(3, 1)
+        p.build_implicit_gemm()
+        return p
+
+    pack = kmap()
+    t_kmap = _median_ms(torch, lambda: fvdb.ext.kernel_map(grid._impl.tensors(), grid._impl.tensors(), 1, [3, 3, 3],
+                                                             [1, 1, 1], False, N), 20)
+    res = {}
+    for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16)):
+        xd, wd = x.to(dt), w.to(dt)
+        t_fwd = _median_ms(torch, lambda: pack.sparse_conv_3d(grid.jagged_like(xd), wd, IG), 20)
+        t_step = _median_ms(torch, lambda: kmap().sparse_conv_3d(grid.jagged_like(xd), wd, IG), 10)
+        res[name] = {"forward_ms": t_fwd, "step_ms": t_step, "voxels_per_s": N / (t_step * 1e-3)}
+    P = int((pack.out_in_map >= 0).sum())
+    ijk = grid.ijk.jdata.cpu().numpy()
+    km = oracle.kernel_map([ijk], [ijk], 3, 1)
+    bit_exact = bool(np.array_equal(pack.out_in_map[:N].cpu().numpy(), km))
+    ref = oracle.sparse_conv(x.double().cpu(), w.double().cpu(), km, N, 1)
+    out = pack.sparse_conv_3d(grid.jagged_like(x), w, IG).jdata
+    err = float((out.double().cpu() - ref).abs().max() / ref.abs().max())
+    kmap_bytes = N * (12 + 4 * 27) + 80 * grid.total_leaf_nodes
+    t0 = time.perf_counter()
+    oracle.kernel_map([ijk], [ijk], 3, 1)
+    t_cpu_kmap = time.perf_counter() - t0
+    return {"workload": "single: 1 grid, 2M points on 8 spheres at 1/256, 3x3x3 32->32", "voxels": N, "pairs": P,
+            "kmap_ms": t_kmap, "kmap_GBps": kmap_bytes / (t_kmap * 1e-3) / 1e9, "kmap_bit_exact": bit_exact,
+            "fp32_max_rel_err_vs_fp64": err, "results": res,
+            "cpu_kmap_ms_1thread": t_cpu_kmap * 1e3}
+
+
+def run_diff(torch, fvdb, dev, args):
+    """~1M voxels (planes + spheres at 1/1024), 3x3x3 128->128 bf16 forward + dgrad + wgrad."""
+    from fvdb.utils.synthetic import planes_spheres_points
+
+    pts = planes_spheres_points(seed=7, n_points=2_400_000, n_planes=2, n_spheres=2, device=dev)
+    grid = fvdb.GridBatch.from_points(pts, voxel_sizes=1 / 1024)
+    N = grid.total_voxels
+    pack, _ = grid.sparse_conv_kernel_map(3, 1)
+    pack.build_implicit_gemm(training=True)
+    pack.build_gather_scatter()
+    P = int(pack.neighborhood_sizes.sum())
+    F = 2.0 * P * 128 * 128
+    g = torch.Generator(device=dev).manual_seed(3)
+    x64 = torch.randn((N, 128), generator=g, device=dev, dtype=torch.float64)
+    w64 = torch.randn((128, 128, 3, 3, 3), generator=g, device=dev, dtype=torch.float64) * math.sqrt(2.0 / (128 * 27))
+    R64 = torch.randn((N, 128), generator=g, device=dev, dtype=torch.float64)
+    IG = fvdb.ConvPackBackend.IGEMM
+    ext = fvdb.ext
+    from fvdb._autograd import weight_to_kio, wgrad_chunk
+
+    out = {}
+    grads = {}
+    pairs, sizes = pack._pairs()
+    chunk = wgrad_chunk(P)
+    for name, dt in (("bf16", torch.bfloat16), ("fp32", torch.float32)):
+        x, w, R = x64.to(dt), w64.to(dt), R64.to(dt)
+        kio = weight_to_kio(w, False)
+        a_f = kio.transpose(1, 2).contiguous()
+        fwd = lambda: ext.implicit_gemm(x, a_f, pack._map_fwd(), N)  # noqa: E731
+        dgr = lambda: ext.implicit_gemm(R, kio, pack._map_bwd(), N)  # noqa: E731
+        wgr = lambda: ext.wgrad(x, R, pairs, sizes, False, chunk)  # noqa: E731
+        t = {"fwd_ms": _median_ms(torch, fwd, 10), "dgrad_ms": _median_ms(torch, dgr, 10),
+             "wgrad_ms": _median_ms(torch, wgr, 10)}
+        t["tflops"] = {k[:-3]: F / (v * 1e-3) / 1e12 for k, v in list(t.items())}
+        # autograd end to end: loss = sum(out * R)
+        xr = x.clone().requires_grad_(True)
+        wr = w.clone().requires_grad_(True)
+
+        def step():
+            y = pack.sparse_conv_3d(grid.jagged_like(xr), wr, IG).jdata
+            (y.float() * R.float()).sum().backward()
+
+        t["train_step_ms"] = _median_ms(torch, step, 10)
+        xr.grad = None
+        wr.grad = None
+        step()
+        grads[name] = (fwd(), xr.grad.detach().clone(), wr.grad.detach().clone())
+        out[name] = t
+        log(f"[bench] diff {name}: {t}")
+
+    # float64 CPU oracle (the reference gather-scatter restated) on the same rounded inputs
+    import oracle
+
+    km = pack.out_in_map[:N].cpu().numpy()
+    errs = {}
+    for name, dt in (("bf16", torch.bfloat16), ("fp32", torch.float32)):
+        xc, wc, Rc = x64.to(dt).double().cpu(), w64.to(dt).double().cpu(), R64.to(dt).double().cpu()
+        ref_out = oracle.sparse_conv(xc, wc, km, N, 1)
+        gi, gw = oracle.sparse_conv_backward(xc, Rc, wc, km, (3, 3, 3))
+
+        def rel(a, b):
+            return float((a.double().cpu() - b).abs().max() / b.abs().max())
+
+        errs[name] = {"out": rel(grads[name][0], ref_out), "dgrad": rel(grads[name][1], gi),
+                      "wgrad": rel(grads[name][2], gw)}
+        log(f"[bench] diff errors {name}: {errs[name]}")
+    # finite differences in fp64 on a ~4k-voxel subgrid, C = 8
+    ijk = grid.ijk.jdata
+    lo = ijk[0]
+    sel = ((ijk - lo).abs() < 40).all(1)
+    sub = fvdb.GridBatch.from_ijk(ijk[sel][:4096].contiguous())
+    sp, _ = sub.sparse_conv_kernel_map(3, 1)
+    sp.build_implicit_gemm(training=True)
+    n = sub.total_voxels
+    gx = torch.Generator(device=dev).manual_seed(11)
+    xs = torch.randn((n, 8), generator=gx, device=dev, dtype=torch.float64, requires_grad=True)
+    ws = (0.2 * torch.randn((8, 8, 3, 3, 3), generator=gx, device=dev, dtype=torch.float64)).requires_grad_(True)
+    Rs = torch.randn((n, 8), generator=gx, device=dev, dtype=torch.float64)
+
+    def loss(a, b):
+        return (sp.sparse_conv_3d(sub.jagged_like(a), b, IG).jdata * Rs).sum()
+
+    loss(xs, ws).backward()
+    h = 1e-6
+    fd_err = 0.0
+    for idx in [(0, 0), (n // 3, 5), (n - 1, 7)]:
+        xp, xm = xs.detach().clone(), xs.detach().clone()
+        xp[idx] += h
+        xm[idx] -= h
+        fd = (loss(xp, ws.detach()) - loss(xm, ws.detach())).item() / (2 * h)
+        fd_err = max(fd_err, abs(fd - xs.grad[idx].item()) / max(1.0, abs(fd)))
+    for idx in [(0, 0, 1, 1, 1), (3, 5, 0, 2, 1), (7, 7, 2, 2, 2)]:
+        wp, wm = ws.detach().clone(), ws.detach().clone()
+        wp[idx] += h
+        wm[idx] -= h
+        fd = (loss(xs.detach(), wp) - loss(xs.detach(), wm)).item() / (2 * h)
+        fd_err = max(fd_err, abs(fd - ws.grad[idx].item()) / max(1.0, abs(fd)))
+    return {"workload": "diff: planes+spheres at 1/1024, 3x3x3 128->128, forward + dgrad + wgrad", "voxels": N,
+            "pairs": P, "flops_per_pass": F, "timings": out, "errors_vs_fp64_cpu_oracle": errs,
+            "fd_check": {"subgrid_voxels": n, "channels": 8, "h": h, "max_rel_err": fd_err, "pass": fd_err < 1e-6}}
+
+
+def _indoor_batch(torch, fvdb, dev, seeds):
+    from fvdb.utils.synthetic import indoor_ijk
+
+    parts = [indoor_ijk(s, device=dev) for s in seeds]
+    return fvdb.GridBatch.from_ijk(fvdb.JaggedTensor(parts), voxel_sizes=0.02)
+
+
+def run_unet(torch, fvdb, dev, args):
+    """4-level sparse U-Net (32/64/128/256), batch of 4 indoor grids, bf16 vs fp32 forward."""
+    grid = _indoor_batch(torch, fvdb, dev, [1, 2, 3, 4])
+    N = grid.total_voxels
+    torch.manual_seed(0)
+    model = fvdb.nn.SparseUNet(in_channels=32, channels=(32, 64, 128, 256)).to(dev).eval()
+    g = torch.Generator(device=dev).manual_seed(5)
+    x = torch.randn((N, 32), generator=g, device=dev)
+    res = {}
+    with torch.no_grad():
+        for name, dt in (("bf16", torch.bfloat16), ("fp32", torch.float32)):
+            m = model.to(dt)
+            inp = fvdb.nn.VDBTensor(grid, grid.jagged_like(x.to(dt)))
+            t = _median_ms(torch, lambda: m(inp), 8, 1)
+            res[name] = {"forward_ms": t, "voxels_per_s": N / (t * 1e-3)}
+            log(f"[bench] unet {name}: {res[name]}")
+        # accuracy against the float64 network (scalar fp64 kernels) on a small room
+        from fvdb.utils.synthetic import indoor_ijk
+
+        small = fvdb.GridBatch.from_ijk(indoor_ijk(9, n_boxes=6, room=(90, 90, 40), device=dev), voxel_sizes=0.02)
+        xs = torch.randn((small.total_voxels, 32), generator=g, device=dev)
+        outs = {}
+        for name, dt in (("bf16", torch.bfloat16), ("fp32", torch.float32), ("fp64", torch.float64)):
+            m = model.to(dt)
+            outs[name] = m(fvdb.nn.VDBTensor(small, small.jagged_like(xs.to(dt)))).data.jdata
+        model.float()
+
+    def rel(a, b):
+        return float((a.double() - b.double()).abs().max() / b.double().abs().max())
+
+    return {"workload": "unet: 4 levels 32/64/128/256, batch of 4 indoor grids at 2 cm", "voxels": N,
+            "voxels_per_grid": grid.num_voxels.tolist(), "results": res,
+            "bf16_over_fp32": res["fp32"]["forward_ms"] / res["bf16"]["forward_ms"],
+            "errors_vs_fp64": {"sample_voxels": small.total_voxels, "bf16": rel(outs["bf16"], outs["fp64"]),
+                               "fp32": rel(outs["fp32"], outs["fp64"])}}
+
+
+def run_serve(torch, fvdb, dev, args):
+    """Poisson stream of single grids (log-uniform 10k-2M voxels), the U-Net in bf16 with continuous
+    batching, against a one-at-a-time fp32 loop over the same stream."""
+    from fvdb.serving import ContinuousBatcher, Request, UNetExecutor, latency_stats
+    from fvdb.utils.synthetic import log_uniform, sphere_points_for_voxels
+
+    nreq = args.requests
+    sizes = [int(v) for v in log_uniform(42, 10_000, 2_000_000, nreq)]
+    reqs_ijk = []
+    for i, nv in enumerate(sizes):
+        pts, vs = sphere_points_for_voxels(1000 + i, nv, device=dev)
+        g = fvdb.GridBatch.from_points(pts, voxel_sizes=vs)
+        reqs_ijk.append(g.ijk.jdata.cpu().pin_memory())
+        del pts, g
+    counts = [t.shape[0] for t in reqs_ijk]
+    torch.manual_seed(0)
+    model = fvdb.nn.SparseUNet(in_channels=32, channels=(32, 64, 128, 256)).to(dev).eval()
+
+    # one-at-a-time fp32 loop (the baseline): requests in arrival order, each alone
+    def one_at_a_time(arrivals, dtype):
+        m = model.to(dtype)
+        t0 = time.perf_counter()
+        lat = []
+        with torch.no_grad():
+            for i, tarr in enumerate(arrivals):
+                while time.perf_counter() - t0 < tarr:
+                    pass
+                grid = fvdb.GridBatch.from_ijk(reqs_ijk[i].to(dev, non_blocking=True))
+                gen = torch.Generator(device=dev).manual_seed(100 + i)
+                xx = torch.randn((grid.total_voxels, 32), generator=gen, device=dev).to(dtype)
+                m(fvdb.nn.VDBTensor(grid, grid.jagged_like(xx)))
+                torch.cuda.synchronize()
+                lat.append(time.perf_counter() - t0 - tarr)
+        model.float()
+        lat.sort()
+        span = time.perf_counter() - t0
+        return {"p50_ms": 1e3 * lat[len(lat) // 2], "p99_ms": 1e3 * lat[min(len(lat) - 1, int(0.99 * len(lat)))],
+                "throughput_voxels_per_s": sum(counts) / span, "throughput_req_per_s": nreq / span, "span_s": span}
+
+    # offered load: measure the bf16 engine's service rate on a closed batch, then run at ~70% of it
+    ex = UNetExecutor(model, dtype=torch.bfloat16)
+    warm = ContinuousBatcher(ex, hbm_budget_bytes=96 << 30)
+    warm.serve([Request(rid=i, ijk=reqs_ijk[i], seed=100 + i) for i in range(min(8, nreq))])
+    t0 = time.perf_counter()
+    closed = ContinuousBatcher(ex, hbm_budget_bytes=96 << 30)
+    closed.serve([Request(rid=i, ijk=reqs_ijk[i], seed=100 + i) for i in range(nreq)])
+    torch.cuda.synchronize()
+    service_rate = nreq / (time.perf_counter() - t0)
+    rate = args.rate if args.rate > 0 else 0.7 * service_rate
+    gexp = torch.Generator().manual_seed(7)
+    gaps = torch.empty(nreq, dtype=torch.float64).exponential_(rate, generator=gexp)
+    arrivals = torch.cumsum(gaps, 0).tolist()
+    batcher = ContinuousBatcher(ex, hbm_budget_bytes=96 << 30, max_batch_voxels=6_000_000, max_inflight=2)
+    t_start = time.perf_counter()
+    done = batcher.serve([Request(rid=i, ijk=reqs_ijk[i], seed=100 + i, arrival=arrivals[i]) for i in range(nreq)])
+    torch.cuda.synchronize()
+    span = time.perf_counter() - t_start
+    ok = [r for r in done if r.status == "done"]
+    stats = latency_stats(done)
+    serve = {"requests": nreq, "completed": len(ok), "rate_req_per_s": rate, "service_rate_closed": service_rate,
+             **stats, "throughput_voxels_per_s": sum(counts[r.rid] for r in ok) / span,
+             "throughput_req_per_s": len(ok) / span, "span_s": span,
+             "mean_batch": statistics.mean(batcher.batches), "max_batch": max(batcher.batches),
+             "peak_reserved_GB": batcher.peak_reserved / 1e9,
+             "hbm_high_water_GB": torch.cuda.max_memory_allocated() / 1e9}
+    base = one_at_a_time(arrivals, torch.float32)
+    return {"workload": "serve: Poisson stream, single grids log-uniform 10k-2M voxels, 4-level U-Net",
+            "voxels_total": sum(counts), "bf16_continuous_batching": serve, "fp32_one_at_a_time": base}
 
 
 if __name__ == "__main__":

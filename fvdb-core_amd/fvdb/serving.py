@@ -1,0 +1,378 @@
+"""Continuous-batching inference server for sparse-conv networks on one GPU.
+
+Requests are single grids (voxel coordinates plus features) that arrive over time.
+
+* **Ingest** (a separate HIP stream): each admitted request's grid index and all the kernel maps its network
+  needs (one 3x3x3 map per level, the stride-2 down map and its transpose between levels) are built once with
+  the gfx950 kernels and stay resident on the device with the request.
+* **Admission**: a request is admitted when its estimated device memory (maps, features, activations, output)
+  fits in the remaining HBM budget; reservations are released when its batch completes or fails.
+* **Batching**: ready requests are packed (FIFO, up to a voxel cap) into one jagged launch per layer. The
+  batch's kernel maps are the per-request maps concatenated with their indices shifted by the request's row
+  offset at that level, so no grid is rebuilt per batch. Launches go to the compute stream while the next
+  requests are ingested on the ingest stream, so their work overlaps on the device.
+* **Results**: every output row of the contraction is computed by one lane in a fixed order, independent of
+  the other rows of the launch (see ``fvdbk_conv_implicit_gemm`` in ``include/fvdbk.h``), and bias/ReLU/concat
+  are per row. A request's output is therefore bit-identical alone and in any mix.
+
+The scheduler (``ContinuousBatcher``) only talks to an executor through ``estimate_bytes``, ``prepare``,
+``launch``, ``is_done`` and ``finish``, so its admission and failure handling can be exercised with a CPU
+executor. ``UNetExecutor`` is the GPU executor for ``fvdb.nn.SparseUNet``.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Any, Callable, Dict, List, Optional, Sequence
+
+import torch
+
+__all__ = ["Request", "ContinuousBatcher", "UNetExecutor", "UNetTopology", "latency_stats"]
+
+
+@dataclass
+class Request:
+    rid: int
+    ijk: Any                      # [N, 3] int32 (host or device)
+    features: Any = None          # [N, C] or None (then generated from `seed` on the device)
+    seed: int = 0
+    arrival: float = 0.0          # seconds on the batcher's clock
+    status: str = "new"           # new -> pending -> prepared -> running -> done | failed
+    error: Optional[BaseException] = None
+    result: Any = None
+    reserved: int = 0
+    t_admit: float = math.nan
+    t_done: float = math.nan
+    state: Any = None             # executor-private (topology, device tensors)
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.arrival
+
+
+class ContinuousBatcher:
+    """Admission by memory budget, FIFO batching, bounded batches in flight."""
+
+    def __init__(self, executor, hbm_budget_bytes: int, max_batch_voxels: int = 4_000_000, max_inflight: int = 2,
+                 clock: Callable[[], float] = time.perf_counter):
+        self.ex = executor
+        self.budget = int(hbm_budget_bytes)
+        self.max_batch_voxels = int(max_batch_voxels)
+        self.max_inflight = int(max_inflight)
+        self.clock = clock
+        self.t0 = clock()
+        self.reserved = 0
+        self.peak_reserved = 0
+        self.queue: List[Request] = []      # arrived, waiting for memory
+        self.prepared: List[Request] = []   # ingested, waiting for a batch
+        self.inflight: List[tuple] = []     # (handle, requests)
+        self.done: List[Request] = []
+        self.batches: List[int] = []        # batch sizes launched
+
+    def now(self) -> float:
+        return self.clock() - self.t0
+
+    def submit(self, req: Request) -> None:
+        try:
+            self.ex.validate(req)
+        except (ValueError, TypeError) as e:
+            self._fail([req], e)
+            return
+        req.status = "pending"
+        self.queue.append(req)
+
+    def _release(self, reqs: Sequence[Request]) -> None:
+        for r in reqs:
+            self.reserved -= r.reserved
+            r.reserved = 0
+
+    def _fail(self, reqs: Sequence[Request], err: BaseException) -> None:
+        self._release(reqs)
+        t = self.now()
+        for r in reqs:
+            r.status = "failed"
+            r.error = err
+            r.t_done = t
+            r.state = None
+            self.done.append(r)
+
+    def _admit(self) -> None:
+        while self.queue:
+            r = self.queue[0]
+            need = self.ex.estimate_bytes(r)
+            if need > self.budget:
+                self.queue.pop(0)
+                self._fail([r], ValueError(f"request {r.rid} needs {need} bytes, over the budget {self.budget}"))
+                continue
+            if self.reserved + need > self.budget:
+                break  # FIFO: wait for memory
+            self.queue.pop(0)
+            r.reserved = need
+            self.reserved += need
+            self.peak_reserved = max(self.peak_reserved, self.reserved)
+            r.t_admit = self.now()
+            try:
+                self.ex.prepare(r)
+            except Exception as e:  # noqa: BLE001 - a bad request fails alone
+                self._fail([r], e)
+                continue
+            r.status = "prepared"
+            self.prepared.append(r)
+
+    def _complete(self) -> None:
+        still = []
+        for handle, reqs in self.inflight:
+            try:
+                ready = self.ex.is_done(handle)
+            except Exception as e:  # noqa: BLE001
+                self._fail(reqs, e)
+                continue
+            if not ready:
+                still.append((handle, reqs))
+                continue
+            try:
+                outs = self.ex.finish(handle)
+            except Exception as e:  # noqa: BLE001
+                self._fail(reqs, e)
+                continue
+            t = self.now()
+            self._release(reqs)
+            for r, o in zip(reqs, outs):
+                r.result = o
+                r.status = "done"
+                r.t_done = t
+                r.state = None
+                self.done.append(r)
+        self.inflight = still
+
+    def _launch(self) -> None:
+        while self.prepared and len(self.inflight) < self.max_inflight:
+            batch, vox = [], 0
+            while self.prepared:
+                n = self.ex.voxels(self.prepared[0])
+                if batch and vox + n > self.max_batch_voxels:
+                    break
+                batch.append(self.prepared.pop(0))
+                vox += n
+            for r in batch:
+                r.status = "running"
+            try:
+                handle = self.ex.launch(batch)
+            except Exception as e:  # noqa: BLE001 - the batch fails, the server goes on
+                self._fail(batch, e)
+                continue
+            self.batches.append(len(batch))
+            self.inflight.append((handle, batch))
+
+    def step(self) -> None:
+        self._complete()
+        self._admit()
+        self._launch()
+
+    def idle(self) -> bool:
+        return not (self.queue or self.prepared or self.inflight)
+
+    def serve(self, requests: Sequence[Request], poll_sleep: float = 0.0) -> List[Request]:
+        """Feed requests at their arrival times (seconds from now) until all are done."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        self.t0 = self.clock()
+        i = 0
+        while i < len(pending) or not self.idle():
+            t = self.now()
+            while i < len(pending) and pending[i].arrival <= t:
+                self.submit(pending[i])
+                i += 1
+            self.step()
+            if poll_sleep:
+                time.sleep(poll_sleep)
+        return sorted(self.done, key=lambda r: r.rid)
+
+
+def latency_stats(reqs: Sequence[Request]) -> Dict[str, float]:
+    lat = sorted(r.latency for r in reqs if r.status == "done")
+    if not lat:
+        return {"p50_ms": math.nan, "p99_ms": math.nan, "mean_ms": math.nan}
+
+    def pct(p):
+        k = min(len(lat) - 1, max(0, int(math.ceil(p / 100.0 * len(lat))) - 1))
+        return lat[k] * 1e3
+
+    return {"p50_ms": pct(50), "p99_ms": pct(99), "mean_ms": 1e3 * sum(lat) / len(lat)}
+
+
+# -----------------------------------------------------------------------------------------------
+# GPU executor for fvdb.nn.SparseUNet
+# -----------------------------------------------------------------------------------------------
+
+
+class UNetTopology:
+    """Per-request device-resident maps of a SparseUNet with L levels.
+
+    n[l]: voxels at level l; conv[l]: [n_l, 27] 3x3x3 map; down[l]: [n_{l+1}, 8] map of the stride-2 conv
+    from level l to l+1 (entries index level l); up[l]: [n_l, 8] transposed map (entries index level l+1).
+    """
+
+    def __init__(self, grid, levels: int):
+        self.n: List[int] = []
+        self.conv: List[torch.Tensor] = []
+        self.down: List[torch.Tensor] = []
+        self.up: List[torch.Tensor] = []
+        g = grid
+        for l in range(levels):
+            n = g.total_voxels
+            self.n.append(n)
+            pc, _ = g.sparse_conv_kernel_map(3, 1)
+            self.conv.append(pc._map_fwd()[:n])
+            if l + 1 < levels:
+                pd, coarse = g.sparse_conv_kernel_map(2, 2)
+                self.down.append(pd._map_fwd()[: coarse.total_voxels])
+                self.up.append(pd._map_bwd()[:n])
+                g = coarse
+
+    def nbytes(self) -> int:
+        return sum(t.numel() * 4 for t in self.conv + self.down + self.up)
+
+
+def _shift_cat(maps: Sequence[torch.Tensor], offsets: Sequence[int]) -> torch.Tensor:
+    """Concatenate maps, adding offsets[i] to the valid entries of maps[i]."""
+    if len(maps) == 1:
+        return maps[0]
+    out = torch.cat(list(maps), 0)
+    rows = torch.tensor([m.shape[0] for m in maps], device=out.device)
+    shift = torch.repeat_interleave(torch.tensor(list(offsets), device=out.device, dtype=torch.int32), rows)
+    return torch.where(out >= 0, out + shift[:, None], out)
+
+
+class UNetExecutor:
+    """Runs a SparseUNet over batches of requests with device-resident per-request maps."""
+
+    def __init__(self, model, dtype=torch.bfloat16, device="cuda", in_channels: Optional[int] = None):
+        import fvdb
+
+        from ._autograd import weight_to_kio
+
+        self.fvdb = fvdb
+        self.device = torch.device(device)
+        self.dtype = dtype
+        self.model = model
+        self.levels = len(model.channels)
+        self.cin = in_channels or model.enc[0][0].in_channels
+        self.ingest_stream = torch.cuda.Stream(device=self.device)
+        self.compute_stream = torch.cuda.Stream(device=self.device)
+
+        def pack(conv):
+            w = conv.weight.detach().to(self.device, dtype)
+            kio = weight_to_kio(w, conv.transposed)  # [K, I, O]
+            b = conv.bias.detach().to(self.device, dtype) if conv.bias is not None else None
+            return (kio.transpose(1, 2).contiguous(), b)
+
+        self.enc = [[pack(c) for c in convs] for convs in model.enc]
+        self.down = [pack(c) for c in model.down]
+        self.up = [pack(c) for c in model.up]
+        self.dec = [[pack(c) for c in convs] for convs in model.dec]
+        self.head = None
+        if model.head is not None:
+            w = model.head.weight.detach().to(self.device, dtype)
+            b = model.head.bias.detach().to(self.device, dtype) if model.head.bias is not None else None
+            self.head = (w, b)
+        self.chans = list(model.channels)
+
+    # -- scheduler interface ----------------------------------------------------------------------
+    def validate(self, req: Request) -> None:
+        ijk = req.ijk
+        if not isinstance(ijk, torch.Tensor) or ijk.dim() != 2 or ijk.shape[1] != 3:
+            raise ValueError(f"request {req.rid}: ijk must be a [N, 3] tensor")
+        if ijk.is_floating_point():
+            raise TypeError(f"request {req.rid}: ijk must be integer")
+        if ijk.shape[0] == 0:
+            raise ValueError(f"request {req.rid}: empty grid")
+        if req.features is not None and (req.features.shape[0] != ijk.shape[0] or req.features.shape[1] != self.cin):
+            raise ValueError(f"request {req.rid}: features must have shape [N, {self.cin}]")
+
+    def voxels(self, req: Request) -> int:
+        return int(req.state["n0"]) if isinstance(req.state, dict) else int(req.ijk.shape[0])
+
+    def estimate_bytes(self, req: Request) -> int:
+        # upper bound before ingest: N0 input voxels; coarse levels at most N0 each (stride-2 surfaces shrink ~4x)
+        n = int(req.ijk.shape[0])
+        el = torch.tensor([], dtype=self.dtype).element_size()
+        per_voxel = 0
+        for l, c in enumerate(self.chans):
+            frac = 1.0 / (4 ** l)
+            per_voxel += frac * ((27 + 8 + 8) * 4 + 6 * c * el + 64)
+        per_voxel += self.cin * el + 12
+        return int(n * per_voxel) + (1 << 20)
+
+    def prepare(self, req: Request) -> None:
+        fvdb = self.fvdb
+        with torch.cuda.stream(self.ingest_stream):
+            ijk = req.ijk.to(self.device, torch.int32, non_blocking=True)
+            grid = fvdb.GridBatch.from_ijk(ijk)
+            topo = UNetTopology(grid, self.levels)
+            if req.features is not None:
+                x = req.features.to(self.device, self.dtype, non_blocking=True)
+            else:
+                gen = torch.Generator(device=self.device).manual_seed(int(req.seed))
+                x = torch.randn((grid.total_voxels, self.cin), generator=gen, device=self.device).to(self.dtype)
+            ev = torch.cuda.Event()
+            ev.record(self.ingest_stream)
+        req.state = {"topo": topo, "x": x, "ready": ev, "n0": grid.total_voxels}
+
+    def launch(self, reqs: Sequence[Request]):
+        ext = self.fvdb.ext
+        with torch.cuda.stream(self.compute_stream):
+            for r in reqs:
+                self.compute_stream.wait_event(r.state["ready"])
+            topos = [r.state["topo"] for r in reqs]
+            L = self.levels
+            offs = [[0] for _ in range(L)]
+            for t in topos:
+                for l in range(L):
+                    offs[l].append(offs[l][-1] + t.n[l])
+            conv = [_shift_cat([t.conv[l] for t in topos], offs[l][:-1]) for l in range(L)]
+            down = [_shift_cat([t.down[l] for t in topos], offs[l][:-1]) for l in range(L - 1)]
+            up = [_shift_cat([t.up[l] for t in topos], offs[l + 1][:-1]) for l in range(L - 1)]
+            x = torch.cat([r.state["x"] for r in reqs], 0) if len(reqs) > 1 else reqs[0].state["x"]
+
+            def layer(inp, wb, m, n_rows):
+                a, b = wb
+                y = ext.implicit_gemm(inp, a, m, n_rows)
+                if b is not None:
+                    y = y + b
+                return torch.relu(y)
+
+            skips = []
+            for l in range(L):
+                n_l = offs[l][-1]
+                if l > 0:
+                    x = layer(x, self.down[l - 1], down[l - 1], n_l)
+                for wb in self.enc[l]:
+                    x = layer(x, wb, conv[l], n_l)
+                skips.append(x)
+            for i, l in enumerate(range(L - 2, -1, -1)):
+                n_l = offs[l][-1]
+                x = layer(x, self.up[i], up[l], n_l)
+                x = torch.cat([x, skips[l]], 1)
+                for wb in self.dec[i]:
+                    x = layer(x, wb, conv[l], n_l)
+            if self.head is not None:
+                w, b = self.head
+                x = x.matmul(w.transpose(0, 1))
+                if b is not None:
+                    x = x + b
+            done = torch.cuda.Event()
+            done.record(self.compute_stream)
+            # tensors made on the ingest stream are read here: tie their memory to this stream too
+            for r in reqs:
+                tp = r.state["topo"]
+                for t in [r.state["x"]] + tp.conv + tp.down + tp.up:
+                    t.record_stream(self.compute_stream)
+        return {"event": done, "out": x, "offs": offs[0]}
+
+    def is_done(self, handle) -> bool:
+        return handle["event"].query()
+
+    def finish(self, handle) -> List[torch.Tensor]:
+        o, out = handle["offs"], handle["out"]
+        return [out[o[i]:o[i + 1]] for i in range(len(o) - 1)]

@@ -4,7 +4,7 @@
   radii U[0.1, 0.3]) -- the single-grid and batched sphere workloads.
 * ``planes_spheres_points``: random planes plus spheres inside the unit cube.
 * ``indoor_ijk``: axis-aligned room walls/floor plus random boxes (surfaces only),
-  voxelised directly on a 2 cm lattice of a 10 x 10 x 3 m room.
+  voxelised directly on a 2 cm lattice of a room of up to 10 x 10 x 3 m.
 
 Points are generated on the device with a seeded ``torch.Generator`` and voxelised by
 ``GridBatch.from_points`` (``ijk = round(p / voxel_size)``).
@@ -59,9 +59,17 @@ def planes_spheres_points(seed: int, n_points: int = 8_000_000, n_planes: int = 
     return torch.cat(parts).float()
 
 
-def indoor_ijk(seed: int, n_boxes: int = 20, room=(500, 500, 150), device="cuda") -> torch.Tensor:
-    """Surface voxels of a room (floor, ceiling, 4 walls) and random boxes, 2 cm lattice."""
+def indoor_ijk(seed: int, n_boxes: int = 20, room=None, device="cuda") -> torch.Tensor:
+    """Surface voxels of a room (floor, ceiling, 4 walls) and random boxes on a 2 cm lattice.
+
+    The room is at most 10 x 10 x 3 m (500 x 500 x 150 cells); without an explicit
+    `room` its footprint is drawn per seed (5-10 m a side), which spreads the voxel
+    counts of a batch over roughly 0.4M-1.2M.
+    """
     g = _gen(seed, device)
+    if room is None:
+        dims = torch.rand(2, generator=g, device=device)
+        room = (int(250 + 250 * dims[0].item()), int(250 + 250 * dims[1].item()), 150)
     X, Y, Z = room
     faces: List[torch.Tensor] = []
 
@@ -91,8 +99,8 @@ def indoor_ijk(seed: int, n_boxes: int = 20, room=(500, 500, 150), device="cuda"
     rect(1, 0, [0, 0], [X - 1, Z - 1])
     rect(1, Y - 1, [0, 0], [X - 1, Z // 2])
     for _ in range(n_boxes):
-        size = (torch.rand(3, generator=g, device=device) * torch.tensor([120.0, 120.0, 100.0], device=device)
-                + 10).int()
+        size = (torch.rand(3, generator=g, device=device) *
+                torch.tensor([X * 0.24, Y * 0.24, 100.0], device=device) + 10).int()
         lo = (torch.rand(3, generator=g, device=device) * (torch.tensor([X, Y, Z], device=device) - size)).int()
         lo[2] = 0 if bool(torch.rand((), generator=g, device=device) < 0.6) else lo[2]
         hi = lo + size
@@ -105,3 +113,22 @@ def log_uniform(seed: int, lo: float, hi: float, n: int) -> List[float]:
     g = torch.Generator().manual_seed(int(seed))
     u = torch.rand(n, generator=g, dtype=torch.float64)
     return torch.exp(math.log(lo) + u * (math.log(hi) - math.log(lo))).tolist()
+
+
+def sphere_points_for_voxels(seed: int, target_voxels: int, n_spheres: int = 8, device="cuda"):
+    """Points on random spheres and a voxel size that gives about `target_voxels` voxels.
+
+    A thin spherical shell of area A voxelised at size v covers about 1.16 A / v^2
+    voxels (measured); 6 A / v^2 points cover it almost completely.
+    Returns (points [n, 3] float32, voxel size).
+    """
+    g = _gen(seed, device)
+    c = torch.rand((n_spheres, 3), generator=g, device=device, dtype=torch.float64)
+    r = 0.1 + 0.2 * torch.rand((n_spheres,), generator=g, device=device, dtype=torch.float64)
+    area = float((4 * math.pi * r ** 2).sum())
+    vs = math.sqrt(1.16 * area / max(int(target_voxels), 1))
+    per = max(1, int(6 * area / vs ** 2) // n_spheres)
+    d = torch.randn((n_spheres, per, 3), generator=g, device=device, dtype=torch.float64)
+    d = d / d.norm(dim=-1, keepdim=True)
+    p = c[:, None, :] + r[:, None, None] * d
+    return p.reshape(-1, 3).float(), vs
