@@ -21,13 +21,13 @@ INCLUDE = os.path.join(ROOT, "include")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("FVDBK_ARCH", "gfx950")
 
-HIP_SOURCES = ["fvdbk_index.hip", "fvdbk_conv.hip"]
+HIP_SOURCES = ["fvdbk_index.hip", "fvdbk_conv.hip", "fvdbk_conv_tile_bf16.hip", "fvdbk_conv_tile_f16.hip",
+               "fvdbk_conv_tile_f32.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-mcode-object-version=5",
     # packed fp32 instructions are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -49,12 +49,22 @@ def ext_filename() -> str:
 
 
 def build_hip() -> str:
+    """Compiles each HIP source to an object (in parallel), then links the library."""
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
+
     out = os.path.join(PKG, "libfvdbk.so")
     tmp = out + ".tmp"
-    cmd = [os.path.join(ROCM, "bin", "hipcc"), *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}"]
-    cmd += [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    cmd += ["-o", tmp]
-    _run(cmd)
+    hipcc = os.path.join(ROCM, "bin", "hipcc")
+    with tempfile.TemporaryDirectory(prefix="fvdbk_build_") as td:
+        objs = [os.path.join(td, os.path.splitext(src)[0] + ".o") for src in HIP_SOURCES]
+        cmds = [[hipcc, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", os.path.join(CSRC, src), "-o", obj]
+                for src, obj in zip(HIP_SOURCES, objs)]
+        jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+        with ThreadPoolExecutor(jobs) as pool:
+            for f in [pool.submit(_run, c) for c in cmds]:
+                f.result()
+        _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, out)
     return out
 

@@ -247,8 +247,31 @@ std::vector<at::Tensor> neighbor_map(at::Tensor map, int64_t n_rows) {
     return {pairs, nbsizes};
 }
 
+// Row order by tap mask: stable descending sort of the per-row tap bitmask.
+// Rows ordered by descending tap mask within consecutive windows of `window` rows
+// (0 = one window over all rows); ties keep row order.  Windows keep the rows of a
+// tile spatially close, so the gathers of a tile stay in a small region of the input.
+at::Tensor row_order(at::Tensor map, int64_t n_rows, int64_t window) {
+    require_hip(map, "map");
+    TORCH_CHECK_VALUE(map.dim() == 2 && map.scalar_type() == at::kInt && map.size(0) >= n_rows,
+                      "map must be int32 [>= rows, K]");
+    map = map.contiguous();
+    c10::hip::HIPGuard guard(map.device().index());
+    at::Tensor keys = at::empty({n_rows}, map.options());
+    check_rc(fvdbk_kmap_row_keys(map.data_ptr<int32_t>(), n_rows, (int32_t)map.size(1), keys.data_ptr<int32_t>(),
+                                 stream_of(map)),
+             "kmap_row_keys");
+    TORCH_CHECK_VALUE(window >= 0, "window must be >= 0");
+    at::Tensor comp = (int64_t(0x7fffffff) - keys.to(at::kLong));
+    if (window > 0 && window < n_rows)
+        comp = comp + at::floor_divide(at::arange(n_rows, comp.options()), window) * int64_t(0x80000000);
+    at::Tensor order = std::get<1>(at::sort(comp, /*stable=*/true, /*dim=*/0, /*descending=*/false));
+    return order.to(at::kInt).contiguous();
+}
+
 // out[r][m] = sum_k sum_c in[map[r][k]][c] * a[k][m][c]
-at::Tensor implicit_gemm(at::Tensor in, at::Tensor a, at::Tensor map, int64_t n_rows) {
+at::Tensor implicit_gemm(at::Tensor in, at::Tensor a, at::Tensor map, int64_t n_rows,
+                         c10::optional<at::Tensor> order) {
     require_hip(in, "features");
     TORCH_CHECK_VALUE(in.dim() == 2, "features must have shape [N, C]");
     TORCH_CHECK_VALUE(a.dim() == 3, "weights must have shape [K, M, C]");
@@ -265,6 +288,16 @@ at::Tensor implicit_gemm(at::Tensor in, at::Tensor a, at::Tensor map, int64_t n_
     map = map.contiguous();
     c10::hip::HIPGuard guard(in.device().index());
     at::Tensor out = at::empty({n_rows, a.size(1)}, in.options());
+    if (order.has_value() && order->defined()) {
+        at::Tensor o = order->contiguous();
+        TORCH_CHECK_VALUE(o.scalar_type() == at::kInt && o.numel() == n_rows, "row order must be int32 [rows]");
+        check_rc(fvdbk_conv_implicit_gemm_sorted(to_dtype(in.scalar_type()), in.data_ptr(), in.size(0),
+                                                 (int32_t)in.size(1), a.data_ptr(), map.data_ptr<int32_t>(),
+                                                 o.data_ptr<int32_t>(), n_rows, (int32_t)a.size(0), (int32_t)a.size(1),
+                                                 out.data_ptr(), 0u, stream_of(in)),
+                 "conv_implicit_gemm_sorted");
+        return out;
+    }
     check_rc(fvdbk_conv_implicit_gemm(to_dtype(in.scalar_type()), in.data_ptr(), in.size(0), (int32_t)in.size(1),
                                       a.data_ptr(), map.data_ptr<int32_t>(), n_rows, (int32_t)a.size(0),
                                       (int32_t)a.size(1), out.data_ptr(), 0u, stream_of(in)),
@@ -323,6 +356,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("conv_grid_candidates", &conv_grid_candidates);
     m.def("kernel_map", &kernel_map);
     m.def("neighbor_map", &neighbor_map);
-    m.def("implicit_gemm", &implicit_gemm);
+    m.def("implicit_gemm", &implicit_gemm, pybind11::arg("in"), pybind11::arg("a"), pybind11::arg("map"),
+          pybind11::arg("n_rows"), pybind11::arg("order") = pybind11::none());
+    m.def("row_order", &row_order, py::arg("map"), py::arg("n_rows"), py::arg("window") = 0);
     m.def("wgrad", &wgrad);
 }

@@ -1,0 +1,11 @@
+// Instances of the LDS-tiled implicit-GEMM contraction for __bf16 features.
+#include "fvdbk_conv_tile.h"
+
+namespace fvdbk {
+
+bool conv_tile_dispatch(const __bf16* X, int cin, const __bf16* A, const int32_t* map, const int32_t* perm,
+                        int64_t n_rows, int K, int M, __bf16* Y, hipStream_t s) {
+    return conv_tile_dispatch_t<__bf16>(X, cin, A, map, perm, n_rows, K, M, Y, s);
+}
+
+}  // namespace fvdbk

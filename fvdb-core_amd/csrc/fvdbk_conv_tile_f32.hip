@@ -1,0 +1,11 @@
+// Instances of the LDS-tiled implicit-GEMM contraction for float features.
+#include "fvdbk_conv_tile.h"
+
+namespace fvdbk {
+
+bool conv_tile_dispatch(const float* X, int cin, const float* A, const int32_t* map, const int32_t* perm,
+                        int64_t n_rows, int K, int M, float* Y, hipStream_t s) {
+    return conv_tile_dispatch_t<float>(X, cin, A, map, perm, n_rows, K, M, Y, s);
+}
+
+}  // namespace fvdbk

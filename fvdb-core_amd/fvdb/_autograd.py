@@ -66,10 +66,10 @@ class SparseConvFunction(torch.autograd.Function):
         n_src, n_tgt = pack.source_grid.total_voxels, pack.target_grid.total_voxels
         kio = weight_to_kio(weight, transposed)  # [K, I, O]
         if not transposed:
-            fmap, n_out = pack._map_fwd(), n_tgt
+            fmap, order, n_out = pack._map_fwd(), pack._order_fwd(), n_tgt
         else:
-            fmap, n_out = pack._map_bwd(), n_src
-        out = ext.implicit_gemm(feats.contiguous(), kio.transpose(1, 2).contiguous(), fmap, n_out)
+            fmap, order, n_out = pack._map_bwd(), pack._order_bwd(), n_src
+        out = ext.implicit_gemm(feats.contiguous(), kio.transpose(1, 2).contiguous(), fmap, n_out, order)
         ctx.save_for_backward(feats, kio)
         ctx.pack = pack
         ctx.transposed = transposed
@@ -85,10 +85,10 @@ class SparseConvFunction(torch.autograd.Function):
         gin = gw = None
         if ctx.needs_input_grad[0]:
             if not transposed:
-                dmap, n_in = pack._map_bwd(), pack.source_grid.total_voxels
+                dmap, order, n_in = pack._map_bwd(), pack._order_bwd(), pack.source_grid.total_voxels
             else:
-                dmap, n_in = pack._map_fwd(), pack.target_grid.total_voxels
-            gin = ext.implicit_gemm(grad_out, kio, dmap, n_in)
+                dmap, order, n_in = pack._map_fwd(), pack._order_fwd(), pack.target_grid.total_voxels
+            gin = ext.implicit_gemm(grad_out, kio, dmap, n_in, order)
         if ctx.needs_input_grad[1]:
             pairs, sizes = pack._pairs()
             gk = ext.wgrad(feats.contiguous(), grad_out, pairs, sizes, transposed, wgrad_chunk(pairs.shape[0]))

@@ -91,6 +91,9 @@ class SparseConvPackInfo:
         self._gs_built = False
         self._use_me = False
         self._igemm = None  # dict of build flags
+        self._sorted = False
+        self._ord_fwd: Optional[torch.Tensor] = None
+        self._ord_bwd: Optional[torch.Tensor] = None
         self._sorted_meta = {}
         self._use_tf32 = False
         self._cutlass = None
@@ -116,6 +119,22 @@ class SparseConvPackInfo:
             self._tmap = ext.kernel_map(self._src._impl.tensors(), self._tgt._impl.tensors(), self._src.grid_count,
                                         list(self._ks), list(self._st), True, _pad128(self._src.total_voxels))
         return self._tmap
+
+    def _order_fwd(self) -> Optional[torch.Tensor]:
+        """Target rows ordered by tap mask (for the sorted contraction), or None."""
+        if not self._sorted:
+            return None
+        if self._ord_fwd is None:
+            self._ord_fwd = ext.row_order(self._map_fwd(), self._tgt.total_voxels)
+        return self._ord_fwd
+
+    def _order_bwd(self) -> Optional[torch.Tensor]:
+        """Source rows of the transposed map ordered by tap mask, or None."""
+        if not self._sorted:
+            return None
+        if self._ord_bwd is None:
+            self._ord_bwd = ext.row_order(self._map_bwd(), self._src.total_voxels)
+        return self._ord_bwd
 
     def _pairs(self):
         if self._pairs_t is None:
@@ -150,6 +169,7 @@ class SparseConvPackInfo:
         self._igemm = {"sorted": bool(sorted), "split": int(split_mask_num), "training": bool(training),
                        "split_bwd": int(split_mask_num_bwd)}
         self._use_tf32 = bool(use_tf32)
+        self._sorted = bool(sorted)
         if sorted:
             self._sorted_meta = self._build_sorted_meta(self._kmap, split_mask_num, self._tgt.total_voxels, 128)
         if training and sorted:

@@ -190,6 +190,21 @@ int fvdbk_conv_implicit_gemm(fvdbk_dtype dtype, const void* in, int64_t n_in, in
                              const int32_t* map, int64_t n_rows, int32_t K, int32_t M, void* out, uint32_t flags,
                              hipStream_t stream);
 
+/* Same contraction with the output rows visited in `row_order` (a permutation of
+ * [0, n_rows)), replacing dispatchSparseConvolutionImplicitGEMMSorted
+ * (backend/SparseConvolutionImplicitGEMMSorted.h, reorder_loc of
+ * SparseConvPackInfo.cpp:117-128).  Rows that share a workgroup share its skipped taps,
+ * so ordering rows by their tap mask (fvdbk_kmap_row_keys, then a stable descending
+ * sort) removes most work on missing neighbours.  Output values are identical to the
+ * unsorted call. */
+int fvdbk_conv_implicit_gemm_sorted(fvdbk_dtype dtype, const void* in, int64_t n_in, int32_t c_in, const void* a,
+                                    const int32_t* map, const int32_t* row_order, int64_t n_rows, int32_t K,
+                                    int32_t M, void* out, uint32_t flags, hipStream_t stream);
+
+/* keys[r] = bit k set when map[r][k] >= 0, for the first min(K, 31) taps (the
+ * reference's bitmask of IGEMMBitOperations.cu:14-36 with one split). */
+int fvdbk_kmap_row_keys(const int32_t* map, int64_t n_rows, int32_t K, int32_t* keys, hipStream_t stream);
+
 /* ---------------------------------------------------------------------------
  * Weight gradient (replaces dispatchSparseConvolutionImplicitGEMMGrad{,Sorted},
  * backend/SparseConvolutionImplicitGEMMGrad.h, and the kernel-gradient half of

@@ -325,3 +325,66 @@ def test_module_and_kmap_cache():
     assert u.grid.is_same(grid) and tuple(u.data.jdata.shape) == (grid.total_voxels, 16)
     (u.data.jdata.sum()).backward()
     assert c1.weight.grad is not None and up.weight.grad is not None
+
+
+@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16, torch.float32])
+@pytest.mark.parametrize("cin,cout", [(64, 64), (32, 128), (20, 24)])
+def test_sorted_rows_bit_exact(dtype, cin, cout):
+    """Rows taken in tap-mask order (with per-tile tap skipping) give the unsorted result bit for bit,
+    forward and backward, for a stride-1 3x3x3 conv and a 2x2x2 stride-2 conv and its transpose."""
+    fvdb = _fvdb()
+    z = golden("sphere_k3s1_c8.npz")
+    g = torch.Generator().manual_seed(11)
+    src = grids_from_fixture(fvdb, z)
+    for ks, st in ((3, 1), (2, 2)):
+        pu, tgt = src.sparse_conv_kernel_map(ks, st)
+        pu.build_implicit_gemm(training=True)
+        ps, _ = src.sparse_conv_kernel_map(ks, st)
+        ps.build_implicit_gemm(sorted=True, training=True)
+        # row orders: permutations, masks non-increasing
+        for order, kmap, n in ((ps._order_fwd(), ps._map_fwd(), tgt.total_voxels),
+                               (ps._order_bwd(), ps._map_bwd(), src.total_voxels)):
+            o = order.long().cpu()
+            assert torch.equal(torch.sort(o).values, torch.arange(n))
+            bits = (kmap[:n] >= 0).long().cpu() << torch.arange(kmap.shape[1]).clamp(max=62)
+            m = bits.sum(1)[o]
+            assert bool((m[:-1] >= m[1:]).all())
+        for transposed in (False, True):
+            n_in = tgt.total_voxels if transposed else src.total_voxels
+            shape = (cin, cout, ks, ks, ks) if transposed else (cout, cin, ks, ks, ks)
+            w = (0.1 * torch.randn(*shape, generator=g)).to("cuda", dtype)
+            x = torch.randn(n_in, cin, generator=g).to("cuda", dtype)
+            outs = []
+            for p in (pu, ps):
+                xx = x.clone().requires_grad_(True)
+                ww = w.clone().requires_grad_(True)
+                f = p.sparse_transpose_conv_3d if transposed else p.sparse_conv_3d
+                y = f(xx, ww, fvdb.ConvPackBackend.IGEMM).jdata
+                gy = torch.linspace(-1, 1, y.numel(), device="cuda").reshape(y.shape).to(dtype)
+                y.backward(gy)
+                outs.append((y.detach(), xx.grad, ww.grad))
+            for a, b in zip(*outs):
+                assert torch.equal(a, b)
+
+
+def test_windowed_row_order_bit_exact():
+    """A row order sorted within windows of consecutive rows is a permutation and leaves the result unchanged."""
+    fvdb = _fvdb()
+    z = golden("sphere_k3s1_c8.npz")
+    src = grids_from_fixture(fvdb, z)
+    pack, _ = src.sparse_conv_kernel_map(3, 1)
+    pack.build_implicit_gemm()
+    n = src.total_voxels
+    kmap = pack._map_fwd()
+    g = torch.Generator().manual_seed(3)
+    x = torch.randn(n, 64, generator=g).to("cuda", torch.bfloat16)
+    a = (0.1 * torch.randn(27, 64, 64, generator=g)).to("cuda", torch.bfloat16)
+    ref = fvdb.ext.implicit_gemm(x, a, kmap, n)
+    for win in (1, 100, 512, n + 5):
+        o = fvdb.ext.row_order(kmap, n, win)
+        ol = o.long().cpu()
+        assert torch.equal(torch.sort(ol).values, torch.arange(n))
+        assert bool(((ol // win) == (torch.arange(n) // win)).all())  # stays inside its window
+        assert torch.equal(fvdb.ext.implicit_gemm(x, a, kmap, n, o), ref)
+    with pytest.raises(ValueError):
+        fvdb.ext.row_order(kmap, n, -1)
