@@ -77,8 +77,9 @@ int64_t next_pow2(int64_t x) {
 }  // namespace
 
 // Build a grid batch index from voxel coordinates (duplicates merged, entries with a
-// negative grid index skipped).  Returns [ijk, jidx, grid_counts, leaf_keys,
-// leaf_masks, word_base, hash].
+// negative grid index skipped).  Two device->host reads: (leaf count, out-of-range
+// count) and the grid offsets.  Returns [ijk, jidx, grid offsets (host int64 [B+1]),
+// grid offsets (device), leaf_keys, leaf_masks, word_base, hash].
 std::vector<at::Tensor> build_grid(at::Tensor ijk, at::Tensor jidx, int64_t n_grids) {
     require_hip(ijk, "ijk");
     TORCH_CHECK_VALUE(ijk.dim() == 2 && ijk.size(1) == 3, "ijk must have shape [N, 3]");
@@ -89,10 +90,6 @@ std::vector<at::Tensor> build_grid(at::Tensor ijk, at::Tensor jidx, int64_t n_gr
     if (has_jidx) {
         TORCH_CHECK_VALUE(jidx.numel() == ijk.size(0), "jidx must have one entry per voxel");
         jidx = jidx.to(at::kInt).contiguous();
-        if (jidx.numel() > 0) {
-            TORCH_CHECK_VALUE(jidx.max().item<int32_t>() < n_grids, "grid index out of range for ", n_grids,
-                              " grids");
-        }
     }
     c10::hip::HIPGuard guard(ijk.device().index());
     hipStream_t s = stream_of(ijk);
@@ -100,21 +97,25 @@ std::vector<at::Tensor> build_grid(at::Tensor ijk, at::Tensor jidx, int64_t n_gr
     auto o32 = ijk.options().dtype(at::kInt);
     auto o64 = ijk.options().dtype(at::kLong);
 
-    at::Tensor keys = at::empty({n}, o64);
-    at::Tensor bad = at::zeros({1}, o32);
-    check_rc(fvdbk_grid_leaf_keys(ijk.data_ptr<int32_t>(), has_jidx ? jidx.data_ptr<int32_t>() : nullptr, n,
-                                  reinterpret_cast<uint64_t*>(keys.data_ptr<int64_t>()), bad.data_ptr<int32_t>(), s),
-             "grid_leaf_keys");
-    const int nbad = bad.item<int32_t>();
-    TORCH_CHECK_VALUE(nbad == 0, nbad, " voxel coordinates lie outside [-2^19, 2^19) or have a grid index >= 1024");
-
-    at::Tensor ukeys = std::get<0>(at::_unique(keys, /*sorted=*/true, /*return_inverse=*/false));
-    int64_t L = ukeys.size(0);
-    if (L > 0 && ukeys[L - 1].item<int64_t>() == INT64_MAX) {
-        ukeys = ukeys.slice(0, 0, L - 1);
-        L -= 1;
-    }
-    ukeys = ukeys.contiguous();
+    // unique leaf keys (unsorted) and the out-of-range count: one device->host read
+    const int64_t set_cap = fvdbk_grid_dedupe_capacity(n);
+    at::Tensor set = at::empty({set_cap}, o64);
+    at::Tensor lkeys = at::empty({std::max<int64_t>(n, 1)}, o64);
+    at::Tensor counters = at::zeros({2}, o32);
+    check_rc(fvdbk_grid_leaf_dedupe(ijk.data_ptr<int32_t>(), has_jidx ? jidx.data_ptr<int32_t>() : nullptr, n,
+                                    (int32_t)n_grids, reinterpret_cast<uint64_t*>(set.data_ptr<int64_t>()), set_cap,
+                                    reinterpret_cast<uint64_t*>(lkeys.data_ptr<int64_t>()),
+                                    reinterpret_cast<uint32_t*>(counters.data_ptr<int32_t>()), s),
+             "grid_leaf_dedupe");
+    at::Tensor hc = counters.to(at::kCPU);
+    const int64_t L = hc.data_ptr<int32_t>()[0];
+    const int nbad = hc.data_ptr<int32_t>()[1];
+    TORCH_CHECK_VALUE(nbad == 0, nbad, " voxels have a grid index >= ", n_grids,
+                      " or coordinates outside [-2^19, 2^19)");
+    // tree order of the leaves: sort only the unique keys (keys are < 2^61, so the
+    // signed sort is the unsigned order)
+    at::Tensor ukeys = std::get<0>(at::sort(lkeys.slice(0, 0, L), /*stable=*/false, /*dim=*/0, /*descending=*/false))
+                           .contiguous();
     const int64_t cap = next_pow2(2 * L + 2);
     at::Tensor hash = at::empty({cap, 2}, o64);
     check_rc(fvdbk_grid_hash_build(reinterpret_cast<const uint64_t*>(ukeys.data_ptr<int64_t>()), L,
@@ -130,18 +131,23 @@ std::vector<at::Tensor> build_grid(at::Tensor ijk, at::Tensor jidx, int64_t n_gr
                                     counts.data_ptr<int32_t>(), s),
              "grid_leaf_counts");
     at::Tensor incl = at::cumsum(counts, 0, at::kLong);
-    const int64_t nv = L > 0 ? incl[L - 1].item<int64_t>() : 0;
     at::Tensor start = (incl - counts.to(at::kLong)).contiguous();
+    // grid offsets on the device; one read gives the total and the per-grid counts
+    at::Tensor jo = at::empty({n_grids + 1}, o64);
+    check_rc(fvdbk_grid_offsets(reinterpret_cast<const uint64_t*>(ukeys.data_ptr<int64_t>()), incl.data_ptr<int64_t>(),
+                                L, (int32_t)n_grids, jo.data_ptr<int64_t>(), s),
+             "grid_offsets");
+    at::Tensor jo_host = jo.to(at::kCPU);
+    const int64_t nv = jo_host.data_ptr<int64_t>()[n_grids];
     at::Tensor wbase = at::empty({L, 8}, o32);
     at::Tensor ijk_out = at::empty({nv, 3}, o32);
     at::Tensor jidx_out = at::empty({nv}, o32);
-    at::Tensor gcounts = at::empty({n_grids}, o32);
     check_rc(fvdbk_grid_finalize(reinterpret_cast<const uint64_t*>(ukeys.data_ptr<int64_t>()),
                                  reinterpret_cast<const uint64_t*>(masks.data_ptr<int64_t>()), start.data_ptr<int64_t>(),
                                  L, nv, wbase.data_ptr<int32_t>(), ijk_out.data_ptr<int32_t>(), jidx_out.data_ptr<int32_t>(),
-                                 gcounts.data_ptr<int32_t>(), (int32_t)n_grids, s),
+                                 nullptr, (int32_t)n_grids, s),
              "grid_finalize");
-    return {ijk_out, jidx_out, gcounts, ukeys, masks, wbase, hash};
+    return {ijk_out, jidx_out, jo_host, jo, ukeys, masks, wbase, hash};
 }
 
 at::Tensor ijk_to_index(at::Tensor g_ijk, at::Tensor g_keys, at::Tensor g_masks, at::Tensor g_wbase, at::Tensor g_hash,

@@ -53,6 +53,71 @@ __global__ void k_leaf_keys(const int32_t* __restrict__ ijk, const int32_t* __re
     keys[i] = leaf_key(b, x, y, z);
 }
 
+// Unique leaf keys of the input voxels, unsorted.  A voxel's key is dropped when the
+// lane before it holds the same key (inputs derived from a grid arrive in tree order,
+// so runs are long); the rest claim a slot of an open-addressing set by a 64-bit CAS
+// (kEmptyKey = free).  Each new key is appended to keys_out: a workgroup reserves
+// room for its new keys with one atomic on counters[0] and lanes take consecutive
+// places by a ballot prefix count.  counters[1] counts voxels out of range (grid index
+// >= n_grids or coordinate outside [-2^19, 2^19)); grid index < 0 skips a voxel.
+constexpr uint64_t kEmptyKey = ~0ull;
+
+__global__ __launch_bounds__(256) void k_leaf_dedupe(const int32_t* __restrict__ ijk,
+                                                     const int32_t* __restrict__ bidx, int64_t n, int32_t n_grids,
+                                                     unsigned long long* __restrict__ set, int64_t cap,
+                                                     uint64_t* __restrict__ keys_out,
+                                                     unsigned int* __restrict__ counters) {
+    __shared__ unsigned int s_cnt[4];
+    __shared__ unsigned int s_base;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t key = kSkipKey;
+    if (i < n) {
+        const int32_t b = bidx ? bidx[i] : 0;
+        if (b >= 0) {
+            const int32_t x = ijk[3 * i], y = ijk[3 * i + 1], z = ijk[3 * i + 2];
+            if (b >= n_grids || b >= kMaxGrids || x < kCoordMin || x > kCoordMax || y < kCoordMin ||
+                y > kCoordMax || z < kCoordMin || z > kCoordMax)
+                atomicAdd(&counters[1], 1u);
+            else
+                key = leaf_key(b, x, y, z);
+        }
+    }
+    const uint64_t prev = __shfl_up(key, 1);
+    bool fresh = false;
+    if (key != kSkipKey && (lane == 0 || prev != key)) {
+        uint64_t slot = mix64(key) & (uint64_t)(cap - 1);
+        for (;;) {
+            const unsigned long long old = atomicCAS(&set[slot], kEmptyKey, (unsigned long long)key);
+            if (old == kEmptyKey) {
+                fresh = true;
+                break;
+            }
+            if (old == key) break;
+            slot = (slot + 1) & (uint64_t)(cap - 1);
+        }
+    }
+    const uint64_t ball = __ballot(fresh);
+    if (lane == 0) s_cnt[wave] = (unsigned int)__popcll(ball);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+        s_base = tot ? atomicAdd(&counters[0], tot) : 0u;
+    }
+    __syncthreads();
+    if (fresh) {
+        unsigned int off = s_base;
+        for (int w = 0; w < wave; ++w) off += s_cnt[w];
+        off += (unsigned int)__popcll(ball & ((1ull << lane) - 1ull));
+        keys_out[off] = key;
+    }
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, int64_t n, unsigned long long v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ void k_hash_clear(fvdbk_hslot* __restrict__ h, int64_t cap) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= cap) return;
@@ -179,6 +244,22 @@ __global__ void k_grid_counts(const uint64_t* __restrict__ keys, const int64_t* 
     const int64_t v0 = l0 < n_leaves ? leaf_start[l0] : n_voxels;
     const int64_t v1 = l1 < n_leaves ? leaf_start[l1] : n_voxels;
     counts[b] = (int32_t)(v1 - v0);
+}
+
+// Voxel offsets of the grids from the inclusive scan of the leaf counts:
+// offsets[b] = voxels in leaves of grids < b, b = 0..n_grids (offsets[n_grids] = all).
+__global__ void k_grid_offsets(const uint64_t* __restrict__ keys, const int64_t* __restrict__ leaf_incl,
+                               int64_t n_leaves, int32_t n_grids, int64_t* __restrict__ offsets) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > n_grids) return;
+    const uint64_t key = (uint64_t)b << 51;
+    int64_t lo = 0, hi = n_leaves;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    offsets[b] = lo > 0 ? leaf_incl[lo - 1] : 0;
 }
 
 // ---------------------------------------------------------------------------------
@@ -504,6 +585,40 @@ int fvdbk_grid_leaf_keys(const int32_t* ijk, const int32_t* batch_idx, int64_t n
     FVDBK_REQUIRE(ijk && keys && bad_count, FVDBK_EINVAL, "null pointer");
     hipLaunchKernelGGL(k_leaf_keys, dim3(grid_1d(n, 256)), dim3(256), 0, stream, ijk, batch_idx, n, keys, bad_count);
     FVDBK_LAUNCH_CHECK("k_leaf_keys");
+    return FVDBK_OK;
+}
+
+int64_t fvdbk_grid_dedupe_capacity(int64_t n) {
+    int64_t cap = 1024;
+    while (cap < 2 * n + 2) cap <<= 1;
+    return cap;
+}
+
+int fvdbk_grid_leaf_dedupe(const int32_t* ijk, const int32_t* batch_idx, int64_t n, int32_t n_grids,
+                           uint64_t* set, int64_t set_cap, uint64_t* leaf_keys, uint32_t* counters,
+                           hipStream_t stream) {
+    FVDBK_REQUIRE(n >= 0 && n_grids >= 0, FVDBK_EINVAL, "grid_leaf_dedupe: negative size");
+    FVDBK_REQUIRE(set_cap >= 2 * n + 2 && (set_cap & (set_cap - 1)) == 0, FVDBK_EINVAL,
+                  "grid_leaf_dedupe: set capacity must be a power of two >= 2n+2");
+    FVDBK_REQUIRE(set && counters && (n == 0 || (ijk && leaf_keys)), FVDBK_EINVAL, "grid_leaf_dedupe: null argument");
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_1d(set_cap, 256)), dim3(256), 0, stream,
+                       reinterpret_cast<unsigned long long*>(set), set_cap, (unsigned long long)kEmptyKey);
+    FVDBK_LAUNCH_CHECK("k_fill_u64");
+    if (n == 0) return FVDBK_OK;
+    hipLaunchKernelGGL(k_leaf_dedupe, dim3(grid_1d(n, 256)), dim3(256), 0, stream, ijk, batch_idx, n, n_grids,
+                       reinterpret_cast<unsigned long long*>(set), set_cap, leaf_keys,
+                       reinterpret_cast<unsigned int*>(counters));
+    FVDBK_LAUNCH_CHECK("k_leaf_dedupe");
+    return FVDBK_OK;
+}
+
+int fvdbk_grid_offsets(const uint64_t* leaf_keys, const int64_t* leaf_incl, int64_t n_leaves, int32_t n_grids,
+                       int64_t* offsets, hipStream_t stream) {
+    FVDBK_REQUIRE(n_leaves >= 0 && n_grids >= 0 && n_grids <= kMaxGrids, FVDBK_EINVAL, "grid_offsets: bad size");
+    FVDBK_REQUIRE(offsets && (n_leaves == 0 || (leaf_keys && leaf_incl)), FVDBK_EINVAL, "grid_offsets: null argument");
+    hipLaunchKernelGGL(k_grid_offsets, dim3(grid_1d(n_grids + 1, 64)), dim3(64), 0, stream, leaf_keys, leaf_incl,
+                       n_leaves, n_grids, offsets);
+    FVDBK_LAUNCH_CHECK("k_grid_offsets");
     return FVDBK_OK;
 }
 

@@ -30,7 +30,8 @@ class _GridIndex:
     __slots__ = ("ijk", "jidx", "keys", "masks", "wbase", "hash", "counts", "joffsets", "n_grids", "voxel_sizes",
                  "origins", "device")
 
-    def __init__(self, ijk, jidx, counts_dev, keys, masks, wbase, hash_, n_grids, voxel_sizes, origins):
+    def __init__(self, ijk, jidx, joffsets_host, joffsets, keys, masks, wbase, hash_, n_grids, voxel_sizes,
+                 origins):
         self.ijk = ijk
         self.jidx = jidx
         self.keys = keys
@@ -39,12 +40,8 @@ class _GridIndex:
         self.hash = hash_
         self.n_grids = int(n_grids)
         self.device = ijk.device
-        counts = counts_dev.to("cpu", torch.int64)
-        self.counts = counts
-        jo = torch.zeros(self.n_grids + 1, dtype=torch.int64)
-        if self.n_grids > 0:
-            jo[1:] = torch.cumsum(counts, 0)
-        self.joffsets = jo.to(self.device)
+        self.counts = joffsets_host[1:] - joffsets_host[:-1]  # voxels per grid (host int64)
+        self.joffsets = joffsets
         self.voxel_sizes = voxel_sizes  # float64 CPU [B,3]
         self.origins = origins
 
@@ -65,8 +62,8 @@ def _build_index(ijk: torch.Tensor, jidx: Optional[torch.Tensor], n_grids: int, 
     j = jidx.to(device=ijk.device, dtype=torch.int32).contiguous() if jidx is not None else \
         torch.empty(0, dtype=torch.int32, device=ijk.device)
     out = ext.build_grid(ijk, j, int(n_grids))
-    ijk_o, jidx_o, counts, keys, masks, wbase, hash_ = out
-    return _GridIndex(ijk_o, jidx_o, counts, keys, masks, wbase, hash_, n_grids, voxel_sizes, origins)
+    ijk_o, jidx_o, jo_host, jo, keys, masks, wbase, hash_ = out
+    return _GridIndex(ijk_o, jidx_o, jo_host, jo, keys, masks, wbase, hash_, n_grids, voxel_sizes, origins)
 
 
 def _as_jagged(x) -> JaggedTensor:

@@ -100,6 +100,17 @@ const char* fvdbk_version(void);
 int fvdbk_grid_leaf_keys(const int32_t* ijk, const int32_t* batch_idx, int64_t n, uint64_t* keys,
                          int32_t* bad_count, hipStream_t stream);
 
+/* Steps 1-2 without a sort of every voxel key: the unique leaf keys of the input
+ * voxels, unsorted (the caller sorts only these n_leaves keys).  set is scratch of
+ * set_cap u64 (fvdbk_grid_dedupe_capacity(n)), leaf_keys has room for n keys,
+ * counters is device uint32[2] zeroed by the caller: counters[0] = n_leaves,
+ * counters[1] = voxels with a grid index >= n_grids or a coordinate outside
+ * [-2^19, 2^19).  Grid indices < 0 skip a voxel. */
+int64_t fvdbk_grid_dedupe_capacity(int64_t n);
+int fvdbk_grid_leaf_dedupe(const int32_t* ijk, const int32_t* batch_idx, int64_t n, int32_t n_grids,
+                           uint64_t* set, int64_t set_cap, uint64_t* leaf_keys, uint32_t* counters,
+                           hipStream_t stream);
+
 int fvdbk_grid_hash_build(const uint64_t* leaf_keys, int64_t n_leaves, fvdbk_hslot* hash, int64_t hash_cap,
                           hipStream_t stream);
 
@@ -115,6 +126,12 @@ int fvdbk_grid_leaf_counts(const uint64_t* leaf_masks, int64_t n_leaves, int32_t
 int fvdbk_grid_finalize(const uint64_t* leaf_keys, const uint64_t* leaf_masks, const int64_t* leaf_start,
                         int64_t n_leaves, int64_t n_voxels, int32_t* leaf_word_base, int32_t* ijk_out,
                         int32_t* jidx_out, int32_t* grid_counts, int32_t n_grids, hipStream_t stream);
+
+/* Voxel offsets of the grids, offsets[b] for b = 0..n_grids (offsets[n_grids] = total
+ * voxels), from the sorted leaf keys and the inclusive scan of the leaf counts; lets
+ * the caller read the total and the per-grid counts in one device->host copy. */
+int fvdbk_grid_offsets(const uint64_t* leaf_keys, const int64_t* leaf_incl, int64_t n_leaves, int32_t n_grids,
+                       int64_t* offsets, hipStream_t stream);
 
 /* Linear index of query coordinates (-1 when inactive).  Replaces
  * src/fvdb/detail/ops/IjkToIndex.cu (GridBatch.ijk_to_index, cumulative indices). */
