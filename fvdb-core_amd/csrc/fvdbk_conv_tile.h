@@ -201,15 +201,17 @@ __global__ __launch_bounds__(256) void k_conv_tile(const T* __restrict__ X, cons
 
 // Fully unrolled variant for a compile-time kernel volume KT (27 = 3^3, 8 = 2^3): the
 // pipeline registers (weights two taps ahead, gathered rows LX taps ahead) are indexed
-// by compile-time tap numbers, so the compiler keeps them in place.  Gathers are
-// branch-free: a missing neighbour loads row 0 and is zeroed by a select.
+// by compile-time tap numbers, so the compiler keeps them in place.
 // SKIP (rows taken through a tap-mask ordering `perm`): taps that no row of the tile
 // uses are skipped, gathers and MFMAs; a skipped tap contributes exact zeros, so the
 // result is bit-identical to the unskipped form.  Without SKIP every tap is visited
 // (the tile of consecutive rows almost always uses all of them, and the branches cost
 // more than they save).
-template <typename T, int MI, int NI, int CIN, int KT, int LX, bool SKIP>
-__global__ __launch_bounds__(256) void k_conv_tile_k(const T* __restrict__ X, const T* __restrict__ A,
+// WPE > 0 asks the register allocator for WPE waves per SIMD (128 registers per
+// lane at 4): latency hiding, not registers, limits this kernel where it fits.
+template <typename T, int MI, int NI, int CIN, int KT, int LX, bool SKIP, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, WPE > 0 ? WPE : 8)))
+void k_conv_tile_k(const T* __restrict__ X, const T* __restrict__ A,
                                                      const int32_t* __restrict__ map,
                                                      const int32_t* __restrict__ perm, int64_t n_rows, int M, int n_mt,
                                                      int64_t n_tiles, T* __restrict__ Y) {
@@ -290,15 +292,18 @@ __global__ __launch_bounds__(256) void k_conv_tile_k(const T* __restrict__ X, co
             }
         }
     };
+    // Lanes of missing neighbours are masked off the load (they keep zeros), so the
+    // memory pipeline moves only the rows that exist.
     auto gather = [&](int k, V (&bb)[NI][NCH]) {
 #pragma unroll
         for (int j = 0; j < NI; ++j) {
             const int32_t src = sMap[(wave * NI * 16 + j * 16 + r16) * KT + k];
-            const T* xr = X + (int64_t)(src >= 0 ? src : 0) * CIN + g * F::EPL;
 #pragma unroll
-            for (int ch = 0; ch < NCH; ++ch) {
-                const V v = *reinterpret_cast<const V*>(xr + ch * F::CH);
-                bb[j][ch] = src >= 0 ? v : zero_frag<T>();
+            for (int ch = 0; ch < NCH; ++ch) bb[j][ch] = zero_frag<T>();
+            if (src >= 0) {
+                const T* xr = X + (int64_t)src * CIN + g * F::EPL;
+#pragma unroll
+                for (int ch = 0; ch < NCH; ++ch) bb[j][ch] = *reinterpret_cast<const V*>(xr + ch * F::CH);
             }
         }
     };
@@ -368,20 +373,25 @@ static void launch_tile(const T* X, const T* A, const int32_t* map, const int32_
     const int64_t n_tiles = n_rt * n_mt;
     const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
     const size_t lds = tile_lds_bytes<T, MI, NI, CIN>(K);
-    // gather lookahead of the unrolled kernel: 2 taps for 16-bit types, 1 for fp32
-    constexpr int LX = sizeof(T) == 2 ? 2 : 1;
+    // Gather lookahead and occupancy of the unrolled kernel (measured on MI355X):
+    // 16-bit types up to 64 channels in 64x128 tiles run best one tap ahead at 4
+    // waves per SIMD; wider tiles keep the default allocation, 2 taps ahead for
+    // 16-bit types and 1 for fp32.
+    constexpr bool W4 = sizeof(T) == 2 && (CIN <= 32 || (CIN == 64 && NI == 2) || (CIN == 64 && MI == 1));
+    constexpr int LX = W4 ? 1 : (sizeof(T) == 2 ? 2 : 1);
+    constexpr int WPE = W4 ? 4 : 0;
     if (K == 27 && perm) {
-        hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 27, LX, true>), grid, dim3(256), lds, s, X, A, map, perm,
-                           n_rows, M, n_mt, n_tiles, Y);
+        hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 27, LX, true, WPE>), grid, dim3(256), lds, s, X, A, map,
+                           perm, n_rows, M, n_mt, n_tiles, Y);
     } else if (K == 27) {
-        hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 27, LX, false>), grid, dim3(256), lds, s, X, A, map, perm,
-                           n_rows, M, n_mt, n_tiles, Y);
+        hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 27, LX, false, WPE>), grid, dim3(256), lds, s, X, A, map,
+                           perm, n_rows, M, n_mt, n_tiles, Y);
     } else if (K == 8 && perm) {
-        hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 8, LX, true>), grid, dim3(256), lds, s, X, A, map, perm,
-                           n_rows, M, n_mt, n_tiles, Y);
+        hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 8, LX, true, WPE>), grid, dim3(256), lds, s, X, A, map,
+                           perm, n_rows, M, n_mt, n_tiles, Y);
     } else if (K == 8) {
-        hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 8, LX, false>), grid, dim3(256), lds, s, X, A, map, perm,
-                           n_rows, M, n_mt, n_tiles, Y);
+        hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 8, LX, false, WPE>), grid, dim3(256), lds, s, X, A, map,
+                           perm, n_rows, M, n_mt, n_tiles, Y);
     } else {
         hipLaunchKernelGGL((k_conv_tile<T, MI, NI, CIN, 1>), grid, dim3(256), lds, s, X, A, map, perm, n_rows, K, M,
                            n_mt, n_tiles, Y);
