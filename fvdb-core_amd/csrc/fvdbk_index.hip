@@ -373,17 +373,19 @@ __device__ inline int64_t xcd_swizzle(int64_t bid, int64_t n) {
     return t < n ? t : -1;
 }
 
-constexpr int KMAP_WAVES = 4;          // row leaves per workgroup (one wavefront each)
+// Row leaves per workgroup (one wavefront each).  One: leaves differ widely in size
+// (1..512 voxels), and a workgroup holds its slots until its largest leaf is done.
+constexpr int KMAP_WAVES = 1;
 constexpr int KMAP_STAGE_BYTES = 16384; // LDS per wavefront for the staged neighbourhood
 
-// One wavefront per row-grid leaf, 4 leaves per workgroup.  The probe-grid leaves
+// One wavefront per row-grid leaf.  The probe-grid leaves
 // that the leaf's taps can reach form a small box (nb_cap leaves at most, fixed per
 // launch); each lane stages one of them (hash lookup, then its 8 mask words and word
 // bases) into the wavefront's LDS slice.  The leaf's active voxels are listed with a
 // ballot compaction of its mask (bit position per voxel rank, no ijk reads), and the
 // leaf's block of the map (cnt*K consecutive int32) is written by consecutive lanes.
 // With nb_cap == 0 (box too large to stage) every pair does a hash probe instead.
-__global__ __launch_bounds__(256) void k_kmap(fvdbk_grid rows, fvdbk_grid probe, ProbeGeom g, int nb_cap,
+__global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk_grid probe, ProbeGeom g, int nb_cap,
                                               int32_t* __restrict__ map) {
     extern __shared__ __attribute__((aligned(16))) unsigned char kmap_smem[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
