@@ -2,6 +2,8 @@
 // launch logic.  Included by one source file per element type, which instantiates
 // conv_tile_dispatch_t for that type only, so the instances compile in parallel.
 #pragma once
+#include <type_traits>
+
 #include "fvdbk_common.h"
 #include "fvdbk_conv_frag.h"
 
@@ -359,6 +361,161 @@ void k_conv_tile_k(const T* __restrict__ X, const T* __restrict__ A,
     }
 }
 
+// Wide-channel form (CIN = NCC x CC): each pipeline step is one (tap, channel chunk)
+// pair, chunk fastest, so a step has exactly the registers and LDS of the CC-channel
+// kernel above (weights two steps ahead, gathered rows LX steps ahead, one barrier per
+// step) however wide the input is.  Up to 64 steps are fully unrolled; longer
+// sequences run a loop whose body is 6 steps unrolled (a multiple of the 3-entry weight
+// ring, the 2-entry LDS ring and the LX+1-entry gather ring), which bounds the code
+// size.  Rows may come through a permutation `perm` (no tap skipping here).  The
+// accumulation order per row is taps ascending, then chunks ascending, then MFMA
+// chunks: fixed, so results do not depend on the launch.
+template <typename T, int MI, int NI, int CC, int NCC, int KT, int LX, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, WPE > 0 ? WPE : 8)))
+void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32_t* __restrict__ map,
+                   const int32_t* __restrict__ perm, int64_t n_rows, int M, int n_mt, int64_t n_tiles,
+                   T* __restrict__ Y) {
+    using F = Frag<T>;
+    using V = typename F::vec;
+    constexpr int CIN = CC * NCC;
+    constexpr int KV = KT * NCC;
+    constexpr int U = 6;
+    constexpr int RT = 4 * NI * 16;
+    constexpr int WP = CC + 16 / (int)sizeof(T);
+    constexpr int WT = MI * 16 * WP;
+    constexpr int NCH = CC / F::CH;
+    constexpr int SEGS = MI * 16 * (CC / F::EPL);
+    constexpr int SEG_PER_T = (SEGS + 255) / 256;
+    static_assert(LX >= 1 && U % (LX + 1) == 0, "gather ring must divide the unrolled step count");
+    extern __shared__ __attribute__((aligned(16))) unsigned char tile_smem[];
+    T* sW = reinterpret_cast<T*>(tile_smem);
+    int32_t* sMap = reinterpret_cast<int32_t*>(tile_smem + 2 * WT * sizeof(T));
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int64_t per = (n_tiles + 7) / 8;
+    const int64_t t = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (t >= n_tiles) return;
+    const int64_t row0 = (t / n_mt) * RT;
+    const int m0 = (int)(t % n_mt) * (MI * 16);
+
+    if (perm == nullptr) {
+        const int64_t base = row0 * KT;
+        const int64_t lim = n_rows * KT;
+        for (int e = tid; e < RT * KT; e += 256) sMap[e] = (base + e < lim) ? map[base + e] : -1;
+    } else {
+        for (int e = tid; e < RT * KT; e += 256) {
+            const int i = e / KT, k = e - i * KT;
+            const int64_t r = row0 + i;
+            sMap[e] = r < n_rows ? map[(int64_t)perm[r] * KT + k] : -1;
+        }
+    }
+
+    floatx4 acc[MI][NI];
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NI; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    V w[3][SEG_PER_T];
+    V b[LX + 1][NI][NCH];
+    auto fetch = [&](int v, V (&wr)[SEG_PER_T]) {
+        const int tap = v / NCC, c0 = (v - tap * NCC) * CC;
+#pragma unroll
+        for (int q = 0; q < SEG_PER_T; ++q) {
+            const int sgi = tid + q * 256;
+            const int m = sgi / (CC / F::EPL), c = (sgi - m * (CC / F::EPL)) * F::EPL;
+            const bool ok = sgi < SEGS && m0 + m < M;
+            const V vv =
+                *reinterpret_cast<const V*>(A + ((int64_t)tap * M + (ok ? m0 + m : 0)) * CIN + c0 + (ok ? c : 0));
+            wr[q] = ok ? vv : zero_frag<T>();
+        }
+    };
+    auto store = [&](int buf, const V (&wr)[SEG_PER_T]) {
+#pragma unroll
+        for (int q = 0; q < SEG_PER_T; ++q) {
+            const int sgi = tid + q * 256;
+            if (SEGS % 256 == 0 || sgi < SEGS) {
+                const int m = sgi / (CC / F::EPL), c = (sgi - m * (CC / F::EPL)) * F::EPL;
+                *reinterpret_cast<V*>(sW + buf * WT + m * WP + c) = wr[q];
+            }
+        }
+    };
+    auto gather = [&](int v, V (&bb)[NI][NCH]) {
+        const int tap = v / NCC, c0 = (v - tap * NCC) * CC;
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+            const int32_t src = sMap[(wave * NI * 16 + j * 16 + r16) * KT + tap];
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) bb[j][ch] = zero_frag<T>();
+            if (src >= 0) {
+                const T* xr = X + (int64_t)src * CIN + c0 + g * F::EPL;
+#pragma unroll
+                for (int ch = 0; ch < NCH; ++ch) bb[j][ch] = *reinterpret_cast<const V*>(xr + ch * F::CH);
+            }
+        }
+    };
+    // one step; u = v mod U selects the ring entries at compile time
+    auto step = [&](int v, auto uc) {
+        constexpr int u = decltype(uc)::value;
+        if (v + 2 < KV) fetch(v + 2, w[(u + 2) % 3]);
+        if (v + LX < KV) gather(v + LX, b[(u + LX) % (LX + 1)]);
+        const T* wb = sW + (u % 2) * WT + r16 * WP + g * F::EPL;
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+#pragma unroll
+            for (int i = 0; i < MI; ++i) {
+                const V a = *reinterpret_cast<const V*>(wb + i * 16 * WP + ch * F::CH);
+#pragma unroll
+                for (int j = 0; j < NI; ++j) mfma(a, b[u % (LX + 1)][j][ch], acc[i][j]);
+            }
+        }
+        if (v + 1 < KV) {
+            store((u + 1) % 2, w[(u + 1) % 3]);
+            __syncthreads();
+        }
+    };
+
+    __syncthreads();  // map tile visible
+    fetch(0, w[0]);
+    store(0, w[0]);
+    if (KV > 1) fetch(1, w[1]);
+#pragma unroll
+    for (int v = 0; v < LX && v < KV; ++v) gather(v, b[v]);
+    __syncthreads();
+    auto six = [&](int v0) {
+        step(v0 + 0, std::integral_constant<int, 0>{});
+        if (KV % U == 0 || v0 + 1 < KV) step(v0 + 1, std::integral_constant<int, 1>{});
+        if (KV % U == 0 || v0 + 2 < KV) step(v0 + 2, std::integral_constant<int, 2>{});
+        if (KV % U == 0 || v0 + 3 < KV) step(v0 + 3, std::integral_constant<int, 3>{});
+        if (KV % U == 0 || v0 + 4 < KV) step(v0 + 4, std::integral_constant<int, 4>{});
+        if (KV % U == 0 || v0 + 5 < KV) step(v0 + 5, std::integral_constant<int, 5>{});
+    };
+    if constexpr (KV <= 64) {
+#pragma unroll
+        for (int v0 = 0; v0 < KV; v0 += U) six(v0);
+    } else {
+#pragma unroll 1
+        for (int v0 = 0; v0 < KV; v0 += U) six(v0);
+    }
+
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+        const int64_t ri = row0 + wave * NI * 16 + j * 16 + r16;
+        if (ri >= n_rows) continue;
+        const int64_t r = perm ? (int64_t)perm[ri] : ri;
+        T* __restrict__ y = Y + r * M;
+#pragma unroll
+        for (int i = 0; i < MI; ++i) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int m = m0 + i * 16 + 4 * g + e;
+                if (m < M) y[m] = from_f32<T>(acc[i][j][e]);
+            }
+        }
+    }
+}
+
 template <typename T, int MI, int NI, int CIN>
 static size_t tile_lds_bytes(int K) {
     constexpr int WP = CIN + 16 / (int)sizeof(T);
@@ -417,6 +574,62 @@ static bool try_tile(const T* X, const T* A, const int32_t* map, const int32_t* 
     return true;
 }
 
+// Wide inputs (CIN = NCC x 64) with K = 27 or 8: the chunked kernel, tile shape and
+// occupancy chosen as for 64 input channels.
+template <typename T, int MI, int NI, int NCC>
+static void launch_tile_c(const T* X, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows, int K,
+                          int M, T* Y, hipStream_t s) {
+    constexpr int CC = 64;
+    const int64_t n_rt = (n_rows + 64 * NI - 1) / (64 * NI);
+    const int n_mt = (M + MI * 16 - 1) / (MI * 16);
+    const int64_t n_tiles = n_rt * n_mt;
+    const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
+    const size_t lds = tile_lds_bytes<T, MI, NI, CC>(K);
+    // Occupancy and lookahead measured per shape on MI355X: 16-bit tiles of 128 or 64
+    // rows run at 4 waves per SIMD one step ahead, except 27 taps x 2 chunks, which is
+    // faster with the default allocation two steps ahead; fp32 runs one step ahead.
+    constexpr bool W4_27 = sizeof(T) == 2 && (NI == 2 || MI == 1) && NCC != 2;
+    constexpr bool W4_8 = sizeof(T) == 2 && (NI == 2 || MI == 1);
+    constexpr int LX27 = W4_27 ? 1 : (sizeof(T) == 2 ? 2 : 1);
+    constexpr int LX8 = W4_8 ? 1 : (sizeof(T) == 2 ? 2 : 1);
+    if (K == 27)
+        hipLaunchKernelGGL((k_conv_tile_c<T, MI, NI, CC, NCC, 27, LX27, W4_27 ? 4 : 0>), grid, dim3(256), lds, s, X, A,
+                           map, perm, n_rows, M, n_mt, n_tiles, Y);
+    else
+        hipLaunchKernelGGL((k_conv_tile_c<T, MI, NI, CC, NCC, 8, LX8, W4_8 ? 4 : 0>), grid, dim3(256), lds, s, X, A,
+                           map, perm, n_rows, M, n_mt, n_tiles, Y);
+}
+
+template <typename T, int NCC>
+static bool try_tile_c(const T* X, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows, int K, int M,
+                       T* Y, hipStream_t s) {
+    if (M > 32) launch_tile_c<T, 4, 2, NCC>(X, A, map, perm, n_rows, K, M, Y, s);
+    else if (M > 16) launch_tile_c<T, 2, 4, NCC>(X, A, map, perm, n_rows, K, M, Y, s);
+    else launch_tile_c<T, 1, 4, NCC>(X, A, map, perm, n_rows, K, M, Y, s);
+    return true;
+}
+
+// Looped kernel only (runtime K), for wide inputs with other kernel volumes.
+template <typename T, int CIN>
+static bool try_tile_looped(const T* X, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows, int K,
+                            int M, T* Y, hipStream_t s) {
+    auto go = [&](auto mi, auto ni) {
+        constexpr int MI = decltype(mi)::value, NI = decltype(ni)::value;
+        const int64_t n_rt = (n_rows + 64 * NI - 1) / (64 * NI);
+        const int n_mt = (M + MI * 16 - 1) / (MI * 16);
+        const int64_t n_tiles = n_rt * n_mt;
+        const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
+        const size_t lds = tile_lds_bytes<T, MI, NI, CIN>(K);
+        hipLaunchKernelGGL((k_conv_tile<T, MI, NI, CIN, 1>), grid, dim3(256), lds, s, X, A, map, perm, n_rows, K, M,
+                           n_mt, n_tiles, Y);
+    };
+    if (M > 32 && tile_lds_bytes<T, 4, 2, CIN>(K) <= TILE_LDS_LIMIT) go(std::integral_constant<int, 4>{}, std::integral_constant<int, 2>{});
+    else if (M > 16 && tile_lds_bytes<T, 2, 4, CIN>(K) <= TILE_LDS_LIMIT) go(std::integral_constant<int, 2>{}, std::integral_constant<int, 4>{});
+    else if (tile_lds_bytes<T, 1, 4, CIN>(K) <= TILE_LDS_LIMIT) go(std::integral_constant<int, 1>{}, std::integral_constant<int, 4>{});
+    else return false;
+    return true;
+}
+
 // LDS-tiled path for the channel counts it is compiled for; false if not applicable.
 template <typename T>
 inline bool conv_tile_dispatch_t(const T* X, int cin, const T* A, const int32_t* map, const int32_t* perm,
@@ -425,8 +638,12 @@ inline bool conv_tile_dispatch_t(const T* X, int cin, const T* A, const int32_t*
     switch (cin) {
     case 32: return try_tile<T, 32>(X, A, map, perm, n_rows, K, M, Y, s);
     case 64: return try_tile<T, 64>(X, A, map, perm, n_rows, K, M, Y, s);
-    case 128: return try_tile<T, 128>(X, A, map, perm, n_rows, K, M, Y, s);
-    case 256: return try_tile<T, 256>(X, A, map, perm, n_rows, K, M, Y, s);
+    case 128:
+        if (K == 27 || K == 8) return try_tile_c<T, 2>(X, A, map, perm, n_rows, K, M, Y, s);
+        return try_tile_looped<T, 128>(X, A, map, perm, n_rows, K, M, Y, s);
+    case 256:
+        if (K == 27 || K == 8) return try_tile_c<T, 4>(X, A, map, perm, n_rows, K, M, Y, s);
+        return try_tile_looped<T, 256>(X, A, map, perm, n_rows, K, M, Y, s);
     default: return false;
     }
 }

@@ -165,6 +165,38 @@ def test_backward_vs_oracle(dtype, transposed):
     assert e_gw <= (tol if dtype in (torch.float64, torch.float32) else 2 * tol)
 
 
+@pytest.mark.parametrize("cin,cout", [(128, 128), (256, 64), (128, 24)])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_wide_channels_vs_oracle(cin, cout, dtype):
+    """Input widths run as (tap, 64-channel chunk) steps: forward, dgrad and wgrad against the
+    float64 oracle for a 3x3x3 stride-1 map (27 taps) and a 2x2x2 stride-2 map (8 taps)."""
+    fvdb = _fvdb()
+    z = golden("sphere_k3s1_c8.npz")
+    src = grids_from_fixture(fvdb, z)
+    g = torch.Generator().manual_seed(cin + cout)
+    for ks, st in ((3, 1), (2, 2)):
+        print(f"[case] wide cin={cin} cout={cout} {dtype} ks={ks} stride={st}", flush=True)
+        pack, tgt = src.sparse_conv_kernel_map(ks, st)
+        pack.build_implicit_gemm(training=True)
+        n_in, n_out = src.total_voxels, tgt.total_voxels
+        x = torch.randn(n_in, cin, generator=g, dtype=torch.float64)
+        w = torch.randn(cout, cin, ks, ks, ks, generator=g, dtype=torch.float64) / math.sqrt(cin * ks ** 3)
+        R = torch.randn(n_out, cout, generator=g, dtype=torch.float64)
+        xd = x.to("cuda", dtype).requires_grad_(True)
+        wd = w.to("cuda", dtype).requires_grad_(True)
+        out = pack.sparse_conv_3d(src.jagged_like(xd), wd, fvdb.ConvPackBackend.IGEMM).jdata
+        (out * R.to("cuda", dtype)).sum().backward()
+        xr, wr = xd.detach().double().cpu(), wd.detach().double().cpu()
+        kmap = pack.out_in_map[:n_out].cpu().numpy()
+        ref_out = oracle.sparse_conv(xr, wr, kmap, n_out, (st, st, st))
+        gi, gw = oracle.sparse_conv_backward(xr, R.to(dtype).double(), wr, kmap, (ks, ks, ks))
+        tol = REL_TOL[dtype]
+        e_out, e_gi, e_gw = rel_err(out, ref_out), rel_err(xd.grad, gi), rel_err(wd.grad, gw)
+        print(f"  out {e_out:.2e} dgrad {e_gi:.2e} wgrad {e_gw:.2e}", flush=True)
+        assert e_out <= tol and e_gi <= tol
+        assert e_gw <= (tol if dtype == torch.float32 else 2 * tol)
+
+
 def dense_box_grid(fvdb, dims):
     return fvdb.GridBatch.from_dense(1, dims, device="cuda")
 
@@ -328,7 +360,7 @@ def test_module_and_kmap_cache():
 
 
 @pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16, torch.float32])
-@pytest.mark.parametrize("cin,cout", [(64, 64), (32, 128), (20, 24)])
+@pytest.mark.parametrize("cin,cout", [(64, 64), (32, 128), (20, 24), (128, 64)])
 def test_sorted_rows_bit_exact(dtype, cin, cout):
     """Rows taken in tap-mask order (with per-tile tap skipping) give the unsorted result bit for bit,
     forward and backward, for a stride-1 3x3x3 conv and a 2x2x2 stride-2 conv and its transpose."""
