@@ -157,24 +157,32 @@ static int conv_mfma_dispatch(const T* X, int cin, const T* A, const int32_t* ma
 
 constexpr int WG_ROWS = 64;  // pairs staged per LDS round
 
+// Software pipeline: each thread owns fixed (pair row, channel group)
+// slots of the staged tiles.  While round r is contracted from LDS, the rows of round
+// r+1 are in flight into registers (their pair indices were loaded one round earlier)
+// and the pair indices of round r+2 are loaded; the rows are written to LDS at the
+// start of the next round.  Every global latency is thus covered by one round of MFMA
+// work instead of being paid in full each round.
 template <typename T, int MI, int NI>
-__global__ __launch_bounds__(256) void k_wgrad_mfma(const T* __restrict__ X, int cin, const T* __restrict__ G,
+__global__ __launch_bounds__(256) void k_wgrad_pipe(const T* __restrict__ X, int cin, const T* __restrict__ G,
                                                     int cout, const int32_t* __restrict__ pairs,
                                                     const int64_t* __restrict__ k_offsets,
                                                     const int32_t* __restrict__ chunk_start, int K, int chunk,
                                                     int transposed, float* __restrict__ ws) {
     using F = Frag<T>;
-    constexpr int TM = 2 * MI * 16;  // c tile of the workgroup
-    constexpr int TN = 2 * NI * 16;  // o tile
-    constexpr int PADB = 32;         // bytes of row padding (spreads the 4 lane groups over banks)
+    using V = typename F::vec;
+    constexpr int TM = 2 * MI * 16;
+    constexpr int TN = 2 * NI * 16;
+    constexpr int PADB = 32;
     constexpr int XP = TM + PADB / (int)sizeof(T);
     constexpr int GP = TN + PADB / (int)sizeof(T);
+    constexpr int XSEG = TM / F::EPL, GSEG = TN / F::EPL;
+    constexpr int XPT = (WG_ROWS * XSEG + 255) / 256, GPT = (WG_ROWS * GSEG + 255) / 256;
     __shared__ __attribute__((aligned(16))) T sX[WG_ROWS * XP];
     __shared__ __attribute__((aligned(16))) T sG[WG_ROWS * GP];
 
     const int id = blockIdx.x;
-    // offset of this chunk: last k with chunk_start[k] <= id
-    int lo = 0, hi = K;  // invariant chunk_start[lo] <= id < chunk_start[hi]
+    int lo = 0, hi = K;  // chunk_start[lo] <= id < chunk_start[hi]
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
         if (chunk_start[mid] <= id) lo = mid;
@@ -188,10 +196,11 @@ __global__ __launch_bounds__(256) void k_wgrad_mfma(const T* __restrict__ X, int
     const int c0 = (blockIdx.y % n_ct) * TM;
     const int o0 = (blockIdx.y / n_ct) * TN;
     const int ci = transposed ? 1 : 0, co = 1 - ci;
-
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, r16 = lane & 15;
     const int wm = (wave & 1) * MI * 16, wn = (wave >> 1) * NI * 16;
+    const bool vec_x = (cin % F::EPL) == 0;
+    const bool vec_g = (cout % F::EPL) == 0;
 
     floatx4 acc[MI][NI];
 #pragma unroll
@@ -199,40 +208,72 @@ __global__ __launch_bounds__(256) void k_wgrad_mfma(const T* __restrict__ X, int
 #pragma unroll
         for (int j = 0; j < NI; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    const bool vec_x = (cin % F::EPL) == 0;
-    const bool vec_g = (cout % F::EPL) == 0;
+    int32_t xi[XPT], gi[GPT];  // pair indices of the round after next
+    V xv[XPT], gv[GPT];        // rows of the next round
+    auto load_idx = [&](int64_t p) {
+#pragma unroll
+        for (int q = 0; q < XPT; ++q) {
+            const int e = tid + 256 * q, rr = e / XSEG;
+            xi[q] = (e < WG_ROWS * XSEG && p + rr < p1) ? pairs[2 * (p + rr) + ci] : -1;
+        }
+#pragma unroll
+        for (int q = 0; q < GPT; ++q) {
+            const int e = tid + 256 * q, rr = e / GSEG;
+            gi[q] = (e < WG_ROWS * GSEG && p + rr < p1) ? pairs[2 * (p + rr) + co] : -1;
+        }
+    };
+    auto load_rows = [&]() {
+#pragma unroll
+        for (int q = 0; q < XPT; ++q) {
+            const int e = tid + 256 * q;
+            const int c = c0 + (e % XSEG) * F::EPL;
+            xv[q] = zero_frag<T>();
+            if (xi[q] >= 0) {
+                const T* src = X + (int64_t)xi[q] * cin;
+                xv[q] = vec_x ? load_frag<T, true>(src, c, cin) : load_frag<T, false>(src, c, cin);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < GPT; ++q) {
+            const int e = tid + 256 * q;
+            const int o = o0 + (e % GSEG) * F::EPL;
+            gv[q] = zero_frag<T>();
+            if (gi[q] >= 0) {
+                const T* src = G + (int64_t)gi[q] * cout;
+                gv[q] = vec_g ? load_frag<T, true>(src, o, cout) : load_frag<T, false>(src, o, cout);
+            }
+        }
+    };
+    auto store_rows = [&]() {
+#pragma unroll
+        for (int q = 0; q < XPT; ++q) {
+            const int e = tid + 256 * q;
+            if (e < WG_ROWS * XSEG)
+                *reinterpret_cast<V*>(&sX[(e / XSEG) * XP + (e % XSEG) * F::EPL]) = xv[q];
+        }
+#pragma unroll
+        for (int q = 0; q < GPT; ++q) {
+            const int e = tid + 256 * q;
+            if (e < WG_ROWS * GSEG)
+                *reinterpret_cast<V*>(&sG[(e / GSEG) * GP + (e % GSEG) * F::EPL]) = gv[q];
+        }
+    };
 
+    if (p0 < p1) {
+        load_idx(p0);
+        load_rows();
+        load_idx(p0 + WG_ROWS);
+    }
     for (int64_t p = p0; p < p1; p += WG_ROWS) {
-        // stage rows [p, p + WG_ROWS) of both operands
-        constexpr int XSEG = TM / F::EPL, GSEG = TN / F::EPL;
-        for (int e = threadIdx.x; e < WG_ROWS * XSEG; e += 256) {
-            const int rr = e / XSEG, sg = e - rr * XSEG;
-            const int c = c0 + sg * F::EPL;
-            typename F::vec v = zero_frag<T>();
-            if (p + rr < p1) {
-                const int32_t s = pairs[2 * (p + rr) + ci];
-                const T* src = X + (int64_t)s * cin;
-                if (vec_x) v = load_frag<T, true>(src, c, cin);
-                else v = load_frag<T, false>(src, c, cin);
-            }
-            *reinterpret_cast<typename F::vec*>(&sX[rr * XP + sg * F::EPL]) = v;
-        }
-        for (int e = threadIdx.x; e < WG_ROWS * GSEG; e += 256) {
-            const int rr = e / GSEG, sg = e - rr * GSEG;
-            const int o = o0 + sg * F::EPL;
-            typename F::vec v = zero_frag<T>();
-            if (p + rr < p1) {
-                const int32_t t = pairs[2 * (p + rr) + co];
-                const T* src = G + (int64_t)t * cout;
-                if (vec_g) v = load_frag<T, true>(src, o, cout);
-                else v = load_frag<T, false>(src, o, cout);
-            }
-            *reinterpret_cast<typename F::vec*>(&sG[rr * GP + sg * F::EPL]) = v;
-        }
+        store_rows();
         __syncthreads();
+        if (p + WG_ROWS < p1) {
+            load_rows();                  // rows of round p + WG_ROWS
+            load_idx(p + 2 * WG_ROWS);    // indices of the round after
+        }
 #pragma unroll
         for (int st = 0; st < WG_ROWS / F::CH; ++st) {
-            typename F::vec a[MI], b[NI];
+            V a[MI], b[NI];
 #pragma unroll
             for (int i = 0; i < MI; ++i)
 #pragma unroll
@@ -249,7 +290,6 @@ __global__ __launch_bounds__(256) void k_wgrad_mfma(const T* __restrict__ X, int
         __syncthreads();
     }
 
-    // D[m = c][n = o]: lane holds m = 4g + e, n = r16
     float* __restrict__ out = ws + (int64_t)id * cin * cout;
 #pragma unroll
     for (int i = 0; i < MI; ++i)
@@ -323,7 +363,7 @@ static void launch_wgrad_mfma(const T* X, int cin, const T* G, int cout, const i
                               float* ws, hipStream_t s) {
     constexpr int TM = 2 * MI * 16, TN = 2 * NI * 16;
     const unsigned tiles = (unsigned)(((cin + TM - 1) / TM) * ((cout + TN - 1) / TN));
-    hipLaunchKernelGGL((k_wgrad_mfma<T, MI, NI>), dim3((unsigned)total_chunks, tiles), dim3(256), 0, s, X, cin, G,
+    hipLaunchKernelGGL((k_wgrad_pipe<T, MI, NI>), dim3((unsigned)total_chunks, tiles), dim3(256), 0, s, X, cin, G,
                        cout, pairs, koff, cstart, K, chunk, transposed, ws);
 }
 
