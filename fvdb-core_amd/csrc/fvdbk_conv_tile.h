@@ -534,7 +534,7 @@ static void launch_tile(const T* X, const T* A, const int32_t* map, const int32_
     // 16-bit types up to 64 channels in 64x128 tiles run best one tap ahead at 4
     // waves per SIMD; wider tiles keep the default allocation, 2 taps ahead for
     // 16-bit types and 1 for fp32.
-    constexpr bool W4 = sizeof(T) == 2 && (CIN <= 32 || (CIN == 64 && NI == 2) || (CIN == 64 && MI == 1));
+    constexpr bool W4 = sizeof(T) == 2 && MI <= 4 && (CIN <= 32 || (CIN == 64 && NI == 2) || (CIN == 64 && MI == 1));
     constexpr int LX = W4 ? 1 : (sizeof(T) == 2 ? 2 : 1);
     constexpr int WPE = W4 ? 4 : 0;
     if (K == 27 && perm) {
@@ -562,6 +562,13 @@ constexpr size_t TILE_LDS_LIMIT = 65536;
 template <typename T, int CIN>
 static bool try_tile(const T* X, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows, int K, int M,
                      T* Y, hipStream_t s) {
+    if constexpr (CIN == 64 && sizeof(T) == 2) {
+        // 128-channel tiles for outputs in multiples of 128 channels and 27 taps
+        if (M % 128 == 0 && K == 27) {
+            launch_tile<T, 8, 2, CIN>(X, A, map, perm, n_rows, K, M, Y, s);
+            return true;
+        }
+    }
     if (M > 32 && tile_lds_bytes<T, 4, 2, CIN>(K) <= TILE_LDS_LIMIT) {
         launch_tile<T, 4, 2, CIN>(X, A, map, perm, n_rows, K, M, Y, s);
     } else if (M > 16 && tile_lds_bytes<T, 2, 4, CIN>(K) <= TILE_LDS_LIMIT) {
@@ -585,11 +592,12 @@ static void launch_tile_c(const T* X, const T* A, const int32_t* map, const int3
     const int64_t n_tiles = n_rt * n_mt;
     const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
     const size_t lds = tile_lds_bytes<T, MI, NI, CC>(K);
-    // Occupancy and lookahead measured per shape on MI355X: 16-bit tiles of 128 or 64
-    // rows run at 4 waves per SIMD one step ahead, except 27 taps x 2 chunks, which is
-    // faster with the default allocation two steps ahead; fp32 runs one step ahead.
-    constexpr bool W4_27 = sizeof(T) == 2 && (NI == 2 || MI == 1) && NCC != 2;
-    constexpr bool W4_8 = sizeof(T) == 2 && (NI == 2 || MI == 1);
+    // Occupancy and lookahead measured per shape on MI355X: 16-bit tiles of 64 output
+    // channels run at 4 waves per SIMD one step ahead, except 27 taps x 2 chunks, which
+    // is faster with the default allocation two steps ahead, as are 128-channel tiles;
+    // fp32 runs one step ahead.
+    constexpr bool W4_27 = sizeof(T) == 2 && MI <= 4 && (NI == 2 || MI == 1) && NCC != 2;
+    constexpr bool W4_8 = sizeof(T) == 2 && MI <= 4 && (NI == 2 || MI == 1);
     constexpr int LX27 = W4_27 ? 1 : (sizeof(T) == 2 ? 2 : 1);
     constexpr int LX8 = W4_8 ? 1 : (sizeof(T) == 2 ? 2 : 1);
     if (K == 27)
@@ -600,9 +608,22 @@ static void launch_tile_c(const T* X, const T* A, const int32_t* map, const int3
                            map, perm, n_rows, M, n_mt, n_tiles, Y);
 }
 
+// 16-bit outputs in multiples of 128 channels take 128-channel tiles (each gathered
+// row feeds twice the MFMAs) where that measured faster: two chunks per tap, or 8 taps.
+template <typename T>
+static bool wide_m_tile(int M, int K, int ncc) {
+    return sizeof(T) == 2 && M % 128 == 0 && (ncc == 2 || K == 8);
+}
+
 template <typename T, int NCC>
 static bool try_tile_c(const T* X, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows, int K, int M,
                        T* Y, hipStream_t s) {
+    if constexpr (sizeof(T) == 2) {
+        if (wide_m_tile<T>(M, K, NCC)) {
+            launch_tile_c<T, 8, 2, NCC>(X, A, map, perm, n_rows, K, M, Y, s);
+            return true;
+        }
+    }
     if (M > 32) launch_tile_c<T, 4, 2, NCC>(X, A, map, perm, n_rows, K, M, Y, s);
     else if (M > 16) launch_tile_c<T, 2, 4, NCC>(X, A, map, perm, n_rows, K, M, Y, s);
     else launch_tile_c<T, 1, 4, NCC>(X, A, map, perm, n_rows, K, M, Y, s);

@@ -165,11 +165,12 @@ def test_backward_vs_oracle(dtype, transposed):
     assert e_gw <= (tol if dtype in (torch.float64, torch.float32) else 2 * tol)
 
 
-@pytest.mark.parametrize("cin,cout", [(128, 128), (256, 64), (128, 24)])
+@pytest.mark.parametrize("cin,cout", [(128, 128), (256, 64), (128, 24), (64, 128), (256, 256)])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
 def test_wide_channels_vs_oracle(cin, cout, dtype):
-    """Input widths run as (tap, 64-channel chunk) steps: forward, dgrad and wgrad against the
-    float64 oracle for a 3x3x3 stride-1 map (27 taps) and a 2x2x2 stride-2 map (8 taps)."""
+    """Wide layers (inputs run as (tap, 64-channel chunk) steps, outputs of 128k channels in
+    128-channel tiles): forward, dgrad and wgrad against the float64 oracle for a 3x3x3 stride-1
+    map (27 taps) and a 2x2x2 stride-2 map (8 taps)."""
     fvdb = _fvdb()
     z = golden("sphere_k3s1_c8.npz")
     src = grids_from_fixture(fvdb, z)
@@ -360,7 +361,7 @@ def test_module_and_kmap_cache():
 
 
 @pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16, torch.float32])
-@pytest.mark.parametrize("cin,cout", [(64, 64), (32, 128), (20, 24), (128, 64)])
+@pytest.mark.parametrize("cin,cout", [(64, 64), (32, 128), (20, 24), (128, 64), (64, 128)])
 def test_sorted_rows_bit_exact(dtype, cin, cout):
     """Rows taken in tap-mask order (with per-tile tap skipping) give the unsorted result bit for bit,
     forward and backward, for a stride-1 3x3x3 conv and a 2x2x2 stride-2 conv and its transpose."""
