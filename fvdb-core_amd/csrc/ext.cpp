@@ -225,6 +225,59 @@ at::Tensor kernel_map(std::vector<at::Tensor> rows, std::vector<at::Tensor> prob
     return map;
 }
 
+// Pooling window map: rows = coarse grid, probe = fine grid, window/stride in (x, y, z).
+at::Tensor window_map(std::vector<at::Tensor> rows, std::vector<at::Tensor> probe, int64_t n_grids,
+                      std::vector<int64_t> window, std::vector<int64_t> stride, int64_t n_rows_padded) {
+    TORCH_CHECK_VALUE(rows.size() == 5 && probe.size() == 5, "grid tensors: [ijk, keys, masks, word_base, hash]");
+    require_hip(rows[0], "grid");
+    auto w3 = vec3(window, "pool_factor");
+    auto s3 = vec3(stride, "stride");
+    c10::hip::HIPGuard guard(rows[0].device().index());
+    fvdbk_grid gr = make_grid(rows[0], rows[1], rows[2], rows[3], rows[4], n_grids);
+    fvdbk_grid gp = make_grid(probe[0], probe[1], probe[2], probe[3], probe[4], n_grids);
+    const int64_t K = (int64_t)w3[0] * w3[1] * w3[2];
+    at::Tensor map = at::empty({n_rows_padded, K}, rows[0].options().dtype(at::kInt));
+    check_rc(fvdbk_kmap_window(&gr, &gp, w3.data(), s3.data(), map.data_ptr<int32_t>(), n_rows_padded,
+                               stream_of(rows[0])),
+             "kmap_window");
+    return map;
+}
+
+// Max pooling over a window map: (out [n_rows, C], argmax [n_rows, C] int32).
+std::vector<at::Tensor> pool_max(at::Tensor in, at::Tensor map, int64_t n_rows) {
+    require_hip(in, "features");
+    TORCH_CHECK_VALUE(in.dim() == 2, "features must have shape [N, C]");
+    TORCH_CHECK_VALUE(map.dim() == 2 && map.scalar_type() == at::kInt && map.size(0) >= n_rows,
+                      "map must be int32 [>= rows, K]");
+    in = in.contiguous();
+    map = map.contiguous();
+    c10::hip::HIPGuard guard(in.device().index());
+    at::Tensor out = at::empty({n_rows, in.size(1)}, in.options());
+    at::Tensor arg = at::empty({n_rows, in.size(1)}, in.options().dtype(at::kInt));
+    if (n_rows > 0 && in.size(1) > 0)
+        check_rc(fvdbk_pool_max(to_dtype(in.scalar_type()), in.data_ptr(), (int32_t)in.size(1), map.data_ptr<int32_t>(),
+                                n_rows, (int32_t)map.size(1), out.data_ptr(), arg.data_ptr<int32_t>(), stream_of(in)),
+                 "pool_max");
+    return {out, arg};
+}
+
+// Average pooling over a window map: T(sum) * T(scale).
+at::Tensor pool_avg(at::Tensor in, at::Tensor map, int64_t n_rows, double scale) {
+    require_hip(in, "features");
+    TORCH_CHECK_VALUE(in.dim() == 2, "features must have shape [N, C]");
+    TORCH_CHECK_VALUE(map.dim() == 2 && map.scalar_type() == at::kInt && map.size(0) >= n_rows,
+                      "map must be int32 [>= rows, K]");
+    in = in.contiguous();
+    map = map.contiguous();
+    c10::hip::HIPGuard guard(in.device().index());
+    at::Tensor out = at::empty({n_rows, in.size(1)}, in.options());
+    if (n_rows > 0 && in.size(1) > 0)
+        check_rc(fvdbk_pool_avg(to_dtype(in.scalar_type()), in.data_ptr(), (int32_t)in.size(1), map.data_ptr<int32_t>(),
+                                n_rows, (int32_t)map.size(1), scale, out.data_ptr(), stream_of(in)),
+                 "pool_avg");
+    return out;
+}
+
 // (pairs [P,2] (source index, target row) ordered by (k, row), nbsizes [K]) from a map.
 std::vector<at::Tensor> neighbor_map(at::Tensor map, int64_t n_rows) {
     require_hip(map, "map");
@@ -362,6 +415,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("conv_grid_candidates", &conv_grid_candidates);
     m.def("kernel_map", &kernel_map);
     m.def("neighbor_map", &neighbor_map);
+    m.def("window_map", &window_map);
+    m.def("pool_max", &pool_max);
+    m.def("pool_avg", &pool_avg);
     m.def("implicit_gemm", &implicit_gemm, pybind11::arg("in"), pybind11::arg("a"), pybind11::arg("map"),
           pybind11::arg("n_rows"), pybind11::arg("order") = pybind11::none());
     m.def("row_order", &row_order, py::arg("map"), py::arg("n_rows"), py::arg("window") = 0);

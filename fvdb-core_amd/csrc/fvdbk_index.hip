@@ -737,39 +737,16 @@ int fvdbk_conv_grid_candidates(const int32_t* ijk, const int32_t* jidx, int64_t 
     return FVDBK_OK;
 }
 
-int fvdbk_kmap_build(const fvdbk_grid* rows, const fvdbk_grid* probe, const int32_t ks[3], const int32_t stride[3],
-                     int32_t inverse, int32_t* map, int64_t n_rows_padded, hipStream_t stream) {
-    FVDBK_REQUIRE(rows && probe, FVDBK_EINVAL, "null grid");
-    FVDBK_REQUIRE(map || n_rows_padded == 0, FVDBK_EINVAL, "null kernel map");
-    for (int d = 0; d < 3; ++d) {
-        FVDBK_REQUIRE(ks[d] > 0, FVDBK_EINVAL, "Expect kernel size to be larger than {0,0,0}.");
-        FVDBK_REQUIRE(stride[d] > 0, FVDBK_EINVAL, "Expect stride to be larger than 0.");
-    }
+// Shared launch of the kernel-map kernel for a probe geometry (x, y, z order).
+static int kmap_launch(const fvdbk_grid* rows, const fvdbk_grid* probe, ProbeGeom g, int32_t* map,
+                       int64_t n_rows_padded, hipStream_t stream) {
     FVDBK_REQUIRE(rows->n_grids == probe->n_grids, FVDBK_EINVAL,
                   "source and target grids must hold the same number of grids (%d vs %d)", rows->n_grids,
                   probe->n_grids);
     FVDBK_REQUIRE(n_rows_padded >= rows->n_voxels, FVDBK_EINVAL, "map has fewer rows than the row grid has voxels");
-    const int64_t K = (int64_t)ks[0] * ks[1] * ks[2];
+    const int64_t K = g.K;
     FVDBK_REQUIRE(K <= 4096, FVDBK_EINVAL, "kernel volume %lld too large", (long long)K);
     FVDBK_REQUIRE(rows->n_voxels * K < ((int64_t)1 << 40), FVDBK_EINVAL, "kernel map too large");
-    ProbeGeom g{};
-    g.K = (int)K;
-    // coordinate axis a (x,y,z) uses kernel dim 2-a of the (d,h,w) convention
-    for (int a = 0; a < 3; ++a) {
-        const int kd = 2 - a;
-        const int kstart = (int)floor(-ks[kd] / 2.0 + 1.0);
-        g.ext[a] = ks[kd];
-        if (!inverse) {
-            g.mul[a] = stride[kd];
-            g.div[a] = 1;
-            g.off0[a] = kstart;
-        } else {
-            g.mul[a] = 1;
-            g.div[a] = stride[kd];
-            g.off0[a] = -kstart;
-        }
-    }
-    g.sgn = inverse ? -1 : 1;
     for (int a = 0; a < 3; ++a) {
         g.shift[a] = -1;
         for (int sh = 0; sh < 31; ++sh)
@@ -793,6 +770,55 @@ int fvdbk_kmap_build(const fvdbk_grid* rows, const fvdbk_grid* probe, const int3
         FVDBK_LAUNCH_CHECK("k_fill_i32");
     }
     return FVDBK_OK;
+}
+
+int fvdbk_kmap_build(const fvdbk_grid* rows, const fvdbk_grid* probe, const int32_t ks[3], const int32_t stride[3],
+                     int32_t inverse, int32_t* map, int64_t n_rows_padded, hipStream_t stream) {
+    FVDBK_REQUIRE(rows && probe, FVDBK_EINVAL, "null grid");
+    FVDBK_REQUIRE(map || n_rows_padded == 0, FVDBK_EINVAL, "null kernel map");
+    for (int d = 0; d < 3; ++d) {
+        FVDBK_REQUIRE(ks[d] > 0, FVDBK_EINVAL, "Expect kernel size to be larger than {0,0,0}.");
+        FVDBK_REQUIRE(stride[d] > 0, FVDBK_EINVAL, "Expect stride to be larger than 0.");
+    }
+    ProbeGeom g{};
+    g.K = ks[0] * ks[1] * ks[2];
+    // coordinate axis a (x,y,z) uses kernel dim 2-a of the (d,h,w) convention
+    for (int a = 0; a < 3; ++a) {
+        const int kd = 2 - a;
+        const int kstart = (int)floor(-ks[kd] / 2.0 + 1.0);
+        g.ext[a] = ks[kd];
+        if (!inverse) {
+            g.mul[a] = stride[kd];
+            g.div[a] = 1;
+            g.off0[a] = kstart;
+        } else {
+            g.mul[a] = 1;
+            g.div[a] = stride[kd];
+            g.off0[a] = -kstart;
+        }
+    }
+    g.sgn = inverse ? -1 : 1;
+    return kmap_launch(rows, probe, g, map, n_rows_padded, stream);
+}
+
+int fvdbk_kmap_window(const fvdbk_grid* rows, const fvdbk_grid* probe, const int32_t window[3],
+                      const int32_t stride[3], int32_t* map, int64_t n_rows_padded, hipStream_t stream) {
+    FVDBK_REQUIRE(rows && probe, FVDBK_EINVAL, "null grid");
+    FVDBK_REQUIRE(map || n_rows_padded == 0, FVDBK_EINVAL, "null window map");
+    for (int a = 0; a < 3; ++a) {
+        FVDBK_REQUIRE(window[a] > 0, FVDBK_EINVAL, "pooling_factor must be greater than 0");
+        FVDBK_REQUIRE(stride[a] > 0, FVDBK_EINVAL, "stride must be greater than 0");
+    }
+    ProbeGeom g{};
+    g.K = window[0] * window[1] * window[2];
+    for (int a = 0; a < 3; ++a) {
+        g.ext[a] = window[a];
+        g.mul[a] = stride[a];
+        g.div[a] = 1;
+        g.off0[a] = 0;
+    }
+    g.sgn = 1;
+    return kmap_launch(rows, probe, g, map, n_rows_padded, stream);
 }
 
 int fvdbk_nbmap_block_counts(const int32_t* map, int64_t n_rows, int32_t K, int32_t* counts, hipStream_t stream) {

@@ -384,6 +384,86 @@ class GridBatch:
         return idx.jagged_like(idx.jdata >= 0)
 
     # ----------------------------------------------------------------------------
+    # pooling, nearest upsampling and injection (grid_batch.py:384-486,748-790,1059-1094,
+    # 1465-1497 of the reference)
+    # ----------------------------------------------------------------------------
+    def _voxel_data(self, data, name: str = "data") -> torch.Tensor:
+        if isinstance(data, JaggedTensor):
+            if data.num_tensors != self.grid_count:
+                raise ValueError(f"{name} must have one list per grid")
+            d = data.jdata
+        else:
+            d = data
+        if d.dim() < 2:
+            raise ValueError(f"{name} must have shape (num_voxels, *) with at least one feature dimension")
+        if d.shape[0] != self.total_voxels:
+            raise ValueError(f"{name} must have the same number of voxels as the grid ({self.total_voxels}), "
+                             f"got {d.shape[0]}")
+        require_device(d, name)
+        return d
+
+    def _pool(self, fn, pool_factor, data, stride, coarse_grid):
+        from ._pooling import window_map
+
+        pf = to_vec3i(pool_factor, "pool_factor", positive=True)
+        st = to_vec3i(stride, "stride", non_negative=True)
+        st = tuple(s if s > 0 else p for s, p in zip(st, pf))
+        d = self._voxel_data(data)
+        coarse = coarse_grid if coarse_grid is not None else self.coarsened_grid(st)
+        if coarse.grid_count != self.grid_count:
+            raise ValueError("coarse_grid must hold as many grids as this grid batch")
+        wmap = window_map(coarse, self, pf, st)
+        return coarse.jagged_like(fn(d, wmap, coarse.total_voxels, pf)), coarse
+
+    def max_pool(self, pool_factor, data, stride=0, coarse_grid: Optional["GridBatch"] = None):
+        """Max over each window ``c*stride + [0, pool_factor)`` of voxels (-inf when none)."""
+        from ._pooling import MaxPoolFunction
+
+        return self._pool(lambda d, m, n, pf: MaxPoolFunction.apply(d, m, n), pool_factor, data, stride,
+                          coarse_grid)
+
+    def avg_pool(self, pool_factor, data, stride=0, coarse_grid: Optional["GridBatch"] = None):
+        """Window sum times 1 / prod(pool_factor) (missing voxels count as zero)."""
+        from ._pooling import AvgPoolFunction
+
+        return self._pool(lambda d, m, n, pf: AvgPoolFunction.apply(d, m, n, 1.0 / (pf[0] * pf[1] * pf[2])),
+                          pool_factor, data, stride, coarse_grid)
+
+    def subdivide(self, subdiv_factor, data, mask: Optional[JaggedTensor] = None,
+                  fine_grid: Optional["GridBatch"] = None):
+        """Nearest upsampling: each fine voxel takes its coarse voxel's value (zero when absent)."""
+        from ._pooling import GatherRowsFunction
+
+        f = to_vec3i(subdiv_factor, "subdiv_factor", positive=True)
+        d = self._voxel_data(data)
+        if mask is not None and isinstance(mask, JaggedTensor) and mask.num_tensors != self.grid_count:
+            raise ValueError("mask must have one list per grid")
+        fine = fine_grid if fine_grid is not None else self.subdivided_grid(f, mask)
+        if fine.grid_count != self.grid_count:
+            raise ValueError("fine_grid must hold as many grids as this grid batch")
+        ft = torch.tensor(f, dtype=torch.int32, device=self.device)
+        parent = torch.div(fine._impl.ijk, ft, rounding_mode="floor")
+        idx = ext.ijk_to_index(*self._impl.tensors(), self.grid_count, parent.contiguous(), fine._impl.jidx)
+        return fine.jagged_like(GatherRowsFunction.apply(d, idx)), fine
+
+    def inject_from(self, src_grid: "GridBatch", src, dst=None, default_value=0) -> JaggedTensor:
+        """Copy ``src`` (data of ``src_grid``) onto the voxels of this grid that exist in ``src_grid``."""
+        from ._pooling import InjectFunction
+
+        s = src_grid._voxel_data(src, "src")
+        if src_grid.grid_count != self.grid_count:
+            raise ValueError("Source grid and destination GridBatches must have the same number of grids")
+        if dst is None:
+            d = torch.full((self.total_voxels,) + tuple(s.shape[1:]), default_value, dtype=s.dtype, device=s.device)
+        else:
+            d = self._voxel_data(dst, "dst")
+            if tuple(d.shape[1:]) != tuple(s.shape[1:]):
+                raise ValueError(f"src and dst must have the same element shape, but got src: {tuple(s.shape[1:])}, "
+                                 f"dst: {tuple(d.shape[1:])}")
+        idx = ext.ijk_to_index(*src_grid._impl.tensors(), src_grid.grid_count, self._impl.ijk, self._impl.jidx)
+        return self.jagged_like(InjectFunction.apply(s, d, idx))
+
+    # ----------------------------------------------------------------------------
     # dense I/O (differentiable; used by the dense-equivalence tests)
     # ----------------------------------------------------------------------------
     def _dense_frame(self, min_coord, grid_size):

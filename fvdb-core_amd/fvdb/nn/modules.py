@@ -37,6 +37,94 @@ def fvnn_module(module):
     return module
 
 
+GridOrVDBTensor = Union[GridBatch, VDBTensor]
+
+
+def _ref_grid(ref):
+    if isinstance(ref, VDBTensor):
+        return ref.grid, ref.kmap
+    if isinstance(ref, GridBatch):
+        return ref, None
+    return None, None
+
+
+@fvnn_module
+class MaxPool(nn.Module):
+    """3D max pooling over voxel windows (``fvdb/nn/modules.py:33-69`` of the reference).
+
+    Coarse voxels whose window holds no fine voxel get zero (the reference's ``-inf -> 0``).
+    """
+
+    def __init__(self, kernel_size, stride=None):
+        super().__init__()
+        self.kernel_size = kernel_size
+        self.stride = stride or self.kernel_size
+
+    def forward(self, input: VDBTensor, ref_coarse_data: Optional[GridOrVDBTensor] = None) -> VDBTensor:
+        coarse_grid, coarse_kmap = _ref_grid(ref_coarse_data)
+        new_feature, new_grid = input.grid.max_pool(self.kernel_size, input.data, stride=self.stride,
+                                                    coarse_grid=coarse_grid)
+        jd = new_feature.jdata
+        jd = torch.where(torch.isinf(jd), torch.zeros_like(jd), jd)
+        return VDBTensor(new_grid, new_grid.jagged_like(jd), kmap=coarse_kmap)
+
+    def extra_repr(self) -> str:
+        return f"kernel_size={self.kernel_size}, stride={self.stride}"
+
+
+@fvnn_module
+class AvgPool(nn.Module):
+    """3D average pooling over voxel windows (``fvdb/nn/modules.py:70-101`` of the reference)."""
+
+    def __init__(self, kernel_size, stride=None):
+        super().__init__()
+        self.kernel_size = kernel_size
+        self.stride = stride or self.kernel_size
+
+    def forward(self, input: VDBTensor, ref_coarse_data: Optional[GridOrVDBTensor] = None) -> VDBTensor:
+        coarse_grid, coarse_kmap = _ref_grid(ref_coarse_data)
+        new_feature, new_grid = input.grid.avg_pool(self.kernel_size, input.data, stride=self.stride,
+                                                    coarse_grid=coarse_grid)
+        return VDBTensor(new_grid, new_feature, kmap=coarse_kmap)
+
+    def extra_repr(self) -> str:
+        return f"kernel_size={self.kernel_size}, stride={self.stride}"
+
+
+@fvnn_module
+class UpsamplingNearest(nn.Module):
+    """Nearest upsampling by a scale factor (``fvdb/nn/modules.py:102-129`` of the reference)."""
+
+    def __init__(self, scale_factor):
+        super().__init__()
+        self.scale_factor = scale_factor
+
+    def forward(self, input: VDBTensor, mask: Optional[JaggedTensor] = None,
+                ref_fine_data: Optional[GridOrVDBTensor] = None) -> VDBTensor:
+        fine_grid, fine_kmap = _ref_grid(ref_fine_data)
+        new_feature, new_grid = input.grid.subdivide(self.scale_factor, input.data, mask, fine_grid=fine_grid)
+        return VDBTensor(new_grid, new_feature, kmap=fine_kmap)
+
+    def extra_repr(self) -> str:
+        return f"scale_factor={self.scale_factor}"
+
+
+@fvnn_module
+class FillFromGrid(nn.Module):
+    """Copy a VDBTensor's features onto another grid (``fvdb/nn/modules.py:131-154``)."""
+
+    def __init__(self, default_value: float = 0.0) -> None:
+        super().__init__()
+        self.default_value = default_value
+
+    def forward(self, input: VDBTensor, other_data: Optional[GridOrVDBTensor] = None) -> VDBTensor:
+        other_grid, other_kmap = _ref_grid(other_data)
+        if other_grid is None:
+            return input
+        new_feature = other_grid.inject_from(input.grid, input.data, default_value=self.default_value)
+        return VDBTensor(other_grid, new_feature, kmap=other_kmap)
+
+
 @fvnn_module
 class SparseConv3d(nn.Module):
     CUTLASS_SUPPORTED_CHANNELS = [

@@ -175,6 +175,29 @@ int fvdbk_kmap_build(const fvdbk_grid* rows, const fvdbk_grid* probe, const int3
                      const int32_t stride[3], int32_t inverse, int32_t* map, int64_t n_rows_padded,
                      hipStream_t stream);
 
+/* Window map of the pooling operators (replaces the per-voxel window loops of
+ * src/fvdb/detail/ops/DownsampleGridMaxPool.cu:17-63 and DownsampleGridAvgPool.cu:17-62):
+ * map[c][k] = index of probe voxel  c*stride + (i, j, l)  or -1, where tap
+ * k = (i*window[1] + j)*window[2] + l, i.e. x outermost, as the reference's loops.
+ * window and stride are in (x, y, z) order. */
+int fvdbk_kmap_window(const fvdbk_grid* rows, const fvdbk_grid* probe, const int32_t window[3],
+                      const int32_t stride[3], int32_t* map, int64_t n_rows_padded, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Pooling over a window map (fvdbk_kmap_window; rows = coarse voxels).  Replaces
+ * the forward voxel callbacks of src/fvdb/detail/ops/DownsampleGridMaxPool.cu:17-63
+ * and DownsampleGridAvgPool.cu:17-62.  in [N_fine][channels], out [n_rows][channels].
+ *   fvdbk_pool_max: out = max over the window's voxels (-inf when none); argmax (may be
+ *     NULL) = index of the first voxel in tap order holding the maximum, -1 when none
+ *     (the voxel that receives the gradient, DownsampleGridMaxPool.cu:65-114).
+ *   fvdbk_pool_avg: out = T(sum) * T(scale), the sum in fp32 (fp64 for fp64) in tap
+ *     order; scale = 1 / window volume in the reference.
+ * ------------------------------------------------------------------------- */
+int fvdbk_pool_max(fvdbk_dtype dtype, const void* in, int32_t channels, const int32_t* map, int64_t n_rows,
+                   int32_t K, void* out, int32_t* argmax, hipStream_t stream);
+int fvdbk_pool_avg(fvdbk_dtype dtype, const void* in, int32_t channels, const int32_t* map, int64_t n_rows,
+                   int32_t K, double scale, void* out, hipStream_t stream);
+
 /* ---------------------------------------------------------------------------
  * Gather-scatter neighbour map (replaces SparseConvPackInfo::buildGatherScatter,
  * src/fvdb/SparseConvPackInfo.cpp:61-88): pairs (source index, target row)

@@ -26,6 +26,10 @@ Parity pins (each checked in ``tests/test_oracle.py``):
 * gather-scatter forward/backward: dense ``torch.nn.functional.conv3d`` /
   ``conv_transpose3d`` in float64 on dense boxes (the reference's own test strategy,
   ``tests/unit/test_conv.py:361-461``), and a brute-force triple loop on tiny inputs.
+* pooling window map, max / average pooling, nearest upsampling, injection:
+  ``torch.nn.functional.max_pool3d`` / ``avg_pool3d`` / ``interpolate(mode="nearest")``
+  on dense tensors that hold -inf / 0 / nothing at missing voxels, on dense boxes and
+  random sparse sets, and set intersection for injection.
 """
 from __future__ import annotations
 
@@ -272,3 +276,96 @@ def sparse_conv_backward(in_feat: torch.Tensor, grad_out: torch.Tensor, weight: 
     gi, gk = gs_backward(in_feat.detach().to("cpu", dtype), grad_out.detach().to("cpu", dtype), w,
                          torch.from_numpy(nb), torch.from_numpy(sizes), transposed)
     return gi, kio_grad_to_weight(gk, kernel_size, transposed)
+
+
+# ---------------------------------------------------------------------------------
+# pooling, nearest upsampling, injection (the operators beside the convolution)
+# ---------------------------------------------------------------------------------
+
+def _index_of(grids: Sequence[np.ndarray]):
+    """{(b, i, j, k): cumulative index} over grids given in tree order."""
+    d, base = {}, 0
+    for b, g in enumerate(grids):
+        for n, c in enumerate(np.asarray(g).reshape(-1, 3).tolist()):
+            d[(b, c[0], c[1], c[2])] = base + n
+        base += len(g)
+    return d
+
+
+def window_map(coarse_grids: Sequence[np.ndarray], fine_grids: Sequence[np.ndarray], pool_factor, stride) -> np.ndarray:
+    """map[c][k] = fine index of ``c*stride + (i, j, l)``, k = (i*f1 + j)*f2 + l, or -1.
+
+    Follows the window loops of ``src/fvdb/detail/ops/DownsampleGridMaxPool.cu:40-60``
+    (x outermost); pool_factor and stride in (x, y, z) order."""
+    f = [int(v) for v in _vec3(pool_factor)]
+    s = [int(v) for v in _vec3(stride)]
+    fine = _index_of(fine_grids)
+    rows = []
+    for b, g in enumerate(coarse_grids):
+        for c in np.asarray(g).reshape(-1, 3).tolist():
+            row = []
+            for i in range(f[0]):
+                for j in range(f[1]):
+                    for l in range(f[2]):
+                        row.append(fine.get((b, c[0] * s[0] + i, c[1] * s[1] + j, c[2] * s[2] + l), -1))
+            rows.append(row)
+    K = f[0] * f[1] * f[2]
+    return np.asarray(rows, dtype=np.int32).reshape(-1, K)
+
+
+def max_pool(fine_data: torch.Tensor, wmap: np.ndarray) -> Tuple[torch.Tensor, torch.Tensor]:
+    """(max over each window's voxels, -inf when none; argmax = first voxel holding it)
+    (``DownsampleGridMaxPool.cu:17-63,65-114``)."""
+    x = fine_data.reshape(fine_data.shape[0], -1)
+    n, C = wmap.shape[0], x.shape[1]
+    out = torch.full((n, C), -math.inf, dtype=x.dtype)
+    arg = torch.full((n, C), -1, dtype=torch.int64)
+    for r in range(n):
+        for s_ in wmap[r].tolist():
+            if s_ < 0:
+                continue
+            v = x[s_]
+            better = v > out[r]
+            out[r] = torch.where(better, v, out[r])
+            arg[r] = torch.where(better, torch.full_like(arg[r], s_), arg[r])
+    return out, arg
+
+
+def avg_pool(fine_data: torch.Tensor, wmap: np.ndarray, scale: float) -> torch.Tensor:
+    """T(sum) * T(scale), the sum accumulated in fp32 (fp64 for fp64) over taps in order
+    (``DownsampleGridAvgPool.cu:17-62``)."""
+    x = fine_data.reshape(fine_data.shape[0], -1)
+    acc = torch.float64 if x.dtype == torch.float64 else torch.float32
+    n, C = wmap.shape[0], x.shape[1]
+    out = torch.empty((n, C), dtype=x.dtype)
+    sc = torch.tensor(scale, dtype=x.dtype).to(acc)
+    for r in range(n):
+        s_ = torch.zeros(C, dtype=acc)
+        for i in wmap[r].tolist():
+            if i >= 0:
+                s_ = s_ + x[i].to(acc)
+        out[r] = (s_.to(x.dtype).to(acc) * sc).to(x.dtype)
+    return out
+
+
+def parent_index(fine_grids: Sequence[np.ndarray], coarse_grids: Sequence[np.ndarray], factor) -> np.ndarray:
+    """Index of the coarse voxel floor(ijk / factor) of each fine voxel, or -1
+    (``UpsampleGridNearest.cu:34-50``, integer floor division)."""
+    f = _vec3(factor)
+    coarse = _index_of(coarse_grids)
+    out = []
+    for b, g in enumerate(fine_grids):
+        for c in np.asarray(g).reshape(-1, 3).tolist():
+            p = np.floor_divide(np.asarray(c), f).tolist()
+            out.append(coarse.get((b, p[0], p[1], p[2]), -1))
+    return np.asarray(out, dtype=np.int64)
+
+
+def inject_index(dst_grids: Sequence[np.ndarray], src_grids: Sequence[np.ndarray]) -> np.ndarray:
+    """Source index of each destination voxel present in the source grid, or -1 (``ops/Inject.cu``)."""
+    src = _index_of(src_grids)
+    out = []
+    for b, g in enumerate(dst_grids):
+        for c in np.asarray(g).reshape(-1, 3).tolist():
+            out.append(src.get((b, c[0], c[1], c[2]), -1))
+    return np.asarray(out, dtype=np.int64)
