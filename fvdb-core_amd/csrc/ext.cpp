@@ -330,7 +330,7 @@ at::Tensor row_order(at::Tensor map, int64_t n_rows, int64_t window) {
 
 // out[r][m] = sum_k sum_c in[map[r][k]][c] * a[k][m][c]
 at::Tensor implicit_gemm(at::Tensor in, at::Tensor a, at::Tensor map, int64_t n_rows,
-                         c10::optional<at::Tensor> order) {
+                         c10::optional<at::Tensor> order, c10::optional<at::Tensor> bias, bool relu) {
     require_hip(in, "features");
     TORCH_CHECK_VALUE(in.dim() == 2, "features must have shape [N, C]");
     TORCH_CHECK_VALUE(a.dim() == 3, "weights must have shape [K, M, C]");
@@ -347,6 +347,30 @@ at::Tensor implicit_gemm(at::Tensor in, at::Tensor a, at::Tensor map, int64_t n_
     map = map.contiguous();
     c10::hip::HIPGuard guard(in.device().index());
     at::Tensor out = at::empty({n_rows, a.size(1)}, in.options());
+    const bool has_bias = bias.has_value() && bias->defined();
+    if (has_bias || relu) {
+        // fused epilogue: act(conv + bias) rounded once
+        at::Tensor b;
+        if (has_bias) {
+            b = bias->to(in.scalar_type()).contiguous();
+            TORCH_CHECK_VALUE(b.dim() == 1 && b.size(0) == a.size(1), "bias must have shape [", a.size(1), "]");
+            TORCH_CHECK_VALUE(b.device() == in.device(), "bias must be on the features' device");
+        }
+        const int32_t* ord = nullptr;
+        at::Tensor o;
+        if (order.has_value() && order->defined()) {
+            o = order->contiguous();
+            TORCH_CHECK_VALUE(o.scalar_type() == at::kInt && o.numel() == n_rows, "row order must be int32 [rows]");
+            ord = o.data_ptr<int32_t>();
+        }
+        check_rc(fvdbk_conv_implicit_gemm_fused(to_dtype(in.scalar_type()), in.data_ptr(), in.size(0),
+                                                (int32_t)in.size(1), a.data_ptr(), map.data_ptr<int32_t>(), ord, n_rows,
+                                                (int32_t)a.size(0), (int32_t)a.size(1),
+                                                has_bias ? b.data_ptr() : nullptr, relu ? FVDBK_CONV_RELU : 0u,
+                                                out.data_ptr(), stream_of(in)),
+                 "conv_implicit_gemm_fused");
+        return out;
+    }
     if (order.has_value() && order->defined()) {
         at::Tensor o = order->contiguous();
         TORCH_CHECK_VALUE(o.scalar_type() == at::kInt && o.numel() == n_rows, "row order must be int32 [rows]");
@@ -419,7 +443,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("pool_max", &pool_max);
     m.def("pool_avg", &pool_avg);
     m.def("implicit_gemm", &implicit_gemm, pybind11::arg("in"), pybind11::arg("a"), pybind11::arg("map"),
-          pybind11::arg("n_rows"), pybind11::arg("order") = pybind11::none());
+          pybind11::arg("n_rows"), pybind11::arg("order") = pybind11::none(), pybind11::arg("bias") = pybind11::none(),
+          pybind11::arg("relu") = false);
     m.def("row_order", &row_order, py::arg("map"), py::arg("n_rows"), py::arg("window") = 0);
     m.def("wgrad", &wgrad);
 }

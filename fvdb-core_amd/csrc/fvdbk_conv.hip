@@ -26,7 +26,7 @@ constexpr int CONV_WAVES = 4;
 template <typename T, int MI, int NI, bool VEC>
 __global__ __launch_bounds__(256) void k_conv_mfma(const T* __restrict__ X, int cin, const T* __restrict__ A,
                                                    const int32_t* __restrict__ map, int64_t n_rows, int K, int M,
-                                                   T* __restrict__ Y) {
+                                                   T* __restrict__ Y, const T* __restrict__ bias, int relu) {
     using F = Frag<T>;
     using V = typename F::vec;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const T* __restrict__ X, int 
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int m = m0 + i * 16 + 4 * g + e;
-                if (m < M) y[m] = from_f32<T>(acc[i][j][e]);
+                if (m < M) y[m] = epilogue<T>(acc[i][j][e], bias, m, relu);
             }
         }
     }
@@ -98,7 +98,8 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const T* __restrict__ X, int 
 // Scalar form for any dtype (fp64 accumulates in fp64): one thread per output element.
 template <typename T, typename Acc>
 __global__ void k_conv_generic(const T* __restrict__ X, int cin, const T* __restrict__ A,
-                               const int32_t* __restrict__ map, int64_t n_rows, int K, int M, T* __restrict__ Y) {
+                               const int32_t* __restrict__ map, int64_t n_rows, int K, int M, T* __restrict__ Y,
+                               const T* __restrict__ bias, int relu) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n_rows * M) return;
     const int64_t r = idx / M;
@@ -111,35 +112,38 @@ __global__ void k_conv_generic(const T* __restrict__ X, int cin, const T* __rest
         const T* __restrict__ a = A + ((int64_t)k * M + m) * cin;
         for (int c = 0; c < cin; ++c) acc += (Acc)x[c] * (Acc)a[c];
     }
+    if (bias) acc += (Acc)bias[m];
+    if (relu && !(acc > (Acc)0) && acc == acc) acc = 0;
     Y[idx] = (T)acc;
 }
 
 template <typename T, int MI, bool VEC>
 static void launch_conv_mfma(const T* X, int cin, const T* A, const int32_t* map, int64_t n_rows, int K, int M, T* Y,
-                             hipStream_t s) {
+                             const T* bias, int relu, hipStream_t s) {
     constexpr int NI = 2;
     const dim3 grid((unsigned)((n_rows + CONV_WAVES * NI * 16 - 1) / (CONV_WAVES * NI * 16)),
                     (unsigned)((M + MI * 16 - 1) / (MI * 16)));
-    hipLaunchKernelGGL((k_conv_mfma<T, MI, NI, VEC>), grid, dim3(256), 0, s, X, cin, A, map, n_rows, K, M, Y);
+    hipLaunchKernelGGL((k_conv_mfma<T, MI, NI, VEC>), grid, dim3(256), 0, s, X, cin, A, map, n_rows, K, M, Y, bias,
+                       relu);
 }
 
 template <typename T>
 static int conv_mfma_dispatch(const T* X, int cin, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows,
-                              int K, int M, T* Y, hipStream_t s) {
-    if (conv_tile_dispatch(X, cin, A, map, perm, n_rows, K, M, Y, s)) {
+                              int K, int M, T* Y, const T* bias, int relu, hipStream_t s) {
+    if (conv_tile_dispatch(X, cin, A, map, perm, n_rows, K, M, Y, bias, relu, s)) {
         FVDBK_LAUNCH_CHECK("k_conv_tile");
         return FVDBK_OK;
     }
     const bool vec = (cin % Frag<T>::EPL) == 0 && ((uintptr_t)X % 16) == 0 && ((uintptr_t)A % 16) == 0;
     const int mi = M <= 16 ? 1 : (M <= 32 ? 2 : 4);
     if (vec) {
-        if (mi == 1) launch_conv_mfma<T, 1, true>(X, cin, A, map, n_rows, K, M, Y, s);
-        else if (mi == 2) launch_conv_mfma<T, 2, true>(X, cin, A, map, n_rows, K, M, Y, s);
-        else launch_conv_mfma<T, 4, true>(X, cin, A, map, n_rows, K, M, Y, s);
+        if (mi == 1) launch_conv_mfma<T, 1, true>(X, cin, A, map, n_rows, K, M, Y, bias, relu, s);
+        else if (mi == 2) launch_conv_mfma<T, 2, true>(X, cin, A, map, n_rows, K, M, Y, bias, relu, s);
+        else launch_conv_mfma<T, 4, true>(X, cin, A, map, n_rows, K, M, Y, bias, relu, s);
     } else {
-        if (mi == 1) launch_conv_mfma<T, 1, false>(X, cin, A, map, n_rows, K, M, Y, s);
-        else if (mi == 2) launch_conv_mfma<T, 2, false>(X, cin, A, map, n_rows, K, M, Y, s);
-        else launch_conv_mfma<T, 4, false>(X, cin, A, map, n_rows, K, M, Y, s);
+        if (mi == 1) launch_conv_mfma<T, 1, false>(X, cin, A, map, n_rows, K, M, Y, bias, relu, s);
+        else if (mi == 2) launch_conv_mfma<T, 2, false>(X, cin, A, map, n_rows, K, M, Y, bias, relu, s);
+        else launch_conv_mfma<T, 4, false>(X, cin, A, map, n_rows, K, M, Y, bias, relu, s);
     }
     FVDBK_LAUNCH_CHECK("k_conv_mfma");
     return FVDBK_OK;
@@ -393,8 +397,8 @@ static int wgrad_mfma_dispatch(const T* X, int cin, const T* G, int cout, const 
 }
 
 static int conv_entry(fvdbk_dtype dtype, const void* in, int64_t n_in, int32_t c_in, const void* a,
-                      const int32_t* map, const int32_t* row_order, int64_t n_rows, int32_t K, int32_t M, void* out,
-                      hipStream_t stream) {
+                      const int32_t* map, const int32_t* row_order, int64_t n_rows, int32_t K, int32_t M,
+                      const void* bias, int relu, void* out, hipStream_t stream) {
     FVDBK_REQUIRE(c_in > 0 && M > 0 && K > 0, FVDBK_EINVAL, "channel counts and kernel volume must be positive");
     FVDBK_REQUIRE(n_in >= 0 && n_rows >= 0, FVDBK_EINVAL, "negative row count");
     FVDBK_REQUIRE(n_rows < ((int64_t)1 << 31), FVDBK_EINVAL, "too many output rows");
@@ -405,17 +409,18 @@ static int conv_entry(fvdbk_dtype dtype, const void* in, int64_t n_in, int32_t c
     switch (dtype) {
     case FVDBK_BF16:
         return conv_mfma_dispatch<__bf16>((const __bf16*)in, c_in, (const __bf16*)a, map, row_order, n_rows, K, M,
-                                          (__bf16*)out, stream);
+                                          (__bf16*)out, (const __bf16*)bias, relu, stream);
     case FVDBK_F16:
         return conv_mfma_dispatch<_Float16>((const _Float16*)in, c_in, (const _Float16*)a, map, row_order, n_rows, K,
-                                            M, (_Float16*)out, stream);
+                                            M, (_Float16*)out, (const _Float16*)bias, relu, stream);
     case FVDBK_F32:
         return conv_mfma_dispatch<float>((const float*)in, c_in, (const float*)a, map, row_order, n_rows, K, M,
-                                         (float*)out, stream);
+                                         (float*)out, (const float*)bias, relu, stream);
     case FVDBK_F64: {
         const int64_t n = n_rows * M;
         hipLaunchKernelGGL((k_conv_generic<double, double>), dim3(grid_1d(n, 256)), dim3(256), 0, stream,
-                           (const double*)in, c_in, (const double*)a, map, n_rows, K, M, (double*)out);
+                           (const double*)in, c_in, (const double*)a, map, n_rows, K, M, (double*)out,
+                           (const double*)bias, relu);
         FVDBK_LAUNCH_CHECK("k_conv_generic");
         return FVDBK_OK;
     }
@@ -443,7 +448,7 @@ int fvdbk_conv_implicit_gemm(fvdbk_dtype dtype, const void* in, int64_t n_in, in
                              const int32_t* map, int64_t n_rows, int32_t K, int32_t M, void* out, uint32_t flags,
                              hipStream_t stream) {
     (void)flags;
-    return conv_entry(dtype, in, n_in, c_in, a, map, nullptr, n_rows, K, M, out, stream);
+    return conv_entry(dtype, in, n_in, c_in, a, map, nullptr, n_rows, K, M, nullptr, 0, out, stream);
 }
 
 int fvdbk_conv_implicit_gemm_sorted(fvdbk_dtype dtype, const void* in, int64_t n_in, int32_t c_in, const void* a,
@@ -451,7 +456,15 @@ int fvdbk_conv_implicit_gemm_sorted(fvdbk_dtype dtype, const void* in, int64_t n
                                     int32_t M, void* out, uint32_t flags, hipStream_t stream) {
     (void)flags;
     FVDBK_REQUIRE(row_order, FVDBK_EINVAL, "null row order");
-    return conv_entry(dtype, in, n_in, c_in, a, map, row_order, n_rows, K, M, out, stream);
+    return conv_entry(dtype, in, n_in, c_in, a, map, row_order, n_rows, K, M, nullptr, 0, out, stream);
+}
+
+int fvdbk_conv_implicit_gemm_fused(fvdbk_dtype dtype, const void* in, int64_t n_in, int32_t c_in, const void* a,
+                                   const int32_t* map, const int32_t* row_order, int64_t n_rows, int32_t K, int32_t M,
+                                   const void* bias, uint32_t flags, void* out, hipStream_t stream) {
+    FVDBK_REQUIRE((flags & ~(uint32_t)FVDBK_CONV_RELU) == 0, FVDBK_EINVAL, "unknown epilogue flags 0x%x", flags);
+    return conv_entry(dtype, in, n_in, c_in, a, map, row_order, n_rows, K, M, bias,
+                      (flags & FVDBK_CONV_RELU) ? 1 : 0, out, stream);
 }
 
 int fvdbk_kmap_row_keys(const int32_t* map, int64_t n_rows, int32_t K, int32_t* keys, hipStream_t stream) {

@@ -47,6 +47,15 @@ __device__ inline void mfma(const floatx4& a, const floatx4& b, floatx4& c) {
 
 template <typename T> __device__ inline T from_f32(float v) { return (T)v; }
 
+// Output epilogue of the contraction: optional per-channel bias and ReLU applied to
+// the fp32 accumulator, then one rounding to T (NaN passes the ReLU unchanged).
+template <typename T>
+__device__ inline T epilogue(float v, const T* __restrict__ bias, int m, int relu) {
+    if (bias) v += (float)bias[m];
+    if (relu && !(v > 0.f) && v == v) v = 0.f;
+    return from_f32<T>(v);
+}
+
 template <typename T, bool VEC>
 __device__ inline typename Frag<T>::vec load_frag(const T* __restrict__ p, int c, int cin) {
     using V = typename Frag<T>::vec;

@@ -29,7 +29,7 @@ template <typename T, int MI, int NI, int CIN, int LA>
 __global__ __launch_bounds__(256) void k_conv_tile(const T* __restrict__ X, const T* __restrict__ A,
                                                    const int32_t* __restrict__ map,
                                                    const int32_t* __restrict__ perm, int64_t n_rows, int K, int M,
-                                                   int n_mt, int64_t n_tiles, T* __restrict__ Y) {
+                                                   int n_mt, int64_t n_tiles, T* __restrict__ Y, const T* __restrict__ bias, int relu) {
     using F = Frag<T>;
     using V = typename F::vec;
     constexpr int RT = 4 * NI * 16;                     // rows per workgroup
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void k_conv_tile(const T* __restrict__ X, cons
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int m = m0 + i * 16 + 4 * g + e;
-                if (m < M) y[m] = from_f32<T>(acc[i][j][e]);
+                if (m < M) y[m] = epilogue<T>(acc[i][j][e], bias, m, relu);
             }
         }
     }
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? W
 void k_conv_tile_k(const T* __restrict__ X, const T* __restrict__ A,
                                                      const int32_t* __restrict__ map,
                                                      const int32_t* __restrict__ perm, int64_t n_rows, int M, int n_mt,
-                                                     int64_t n_tiles, T* __restrict__ Y) {
+                                                     int64_t n_tiles, T* __restrict__ Y, const T* __restrict__ bias, int relu) {
     using F = Frag<T>;
     using V = typename F::vec;
     constexpr int RT = 4 * NI * 16;
@@ -355,7 +355,7 @@ void k_conv_tile_k(const T* __restrict__ X, const T* __restrict__ A,
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int m = m0 + i * 16 + 4 * g + e;
-                if (m < M) y[m] = from_f32<T>(acc[i][j][e]);
+                if (m < M) y[m] = epilogue<T>(acc[i][j][e], bias, m, relu);
             }
         }
     }
@@ -374,7 +374,7 @@ template <typename T, int MI, int NI, int CC, int NCC, int KT, int LX, int WPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, WPE > 0 ? WPE : 8)))
 void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32_t* __restrict__ map,
                    const int32_t* __restrict__ perm, int64_t n_rows, int M, int n_mt, int64_t n_tiles,
-                   T* __restrict__ Y) {
+                   T* __restrict__ Y, const T* __restrict__ bias, int relu) {
     using F = Frag<T>;
     using V = typename F::vec;
     constexpr int CIN = CC * NCC;
@@ -510,7 +510,7 @@ void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int m = m0 + i * 16 + 4 * g + e;
-                if (m < M) y[m] = from_f32<T>(acc[i][j][e]);
+                if (m < M) y[m] = epilogue<T>(acc[i][j][e], bias, m, relu);
             }
         }
     }
@@ -524,7 +524,7 @@ static size_t tile_lds_bytes(int K) {
 
 template <typename T, int MI, int NI, int CIN>
 static void launch_tile(const T* X, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows, int K, int M,
-                        T* Y, hipStream_t s) {
+                        T* Y, const T* bias, int relu, hipStream_t s) {
     const int64_t n_rt = (n_rows + 64 * NI - 1) / (64 * NI);
     const int n_mt = (M + MI * 16 - 1) / (MI * 16);
     const int64_t n_tiles = n_rt * n_mt;
@@ -539,19 +539,19 @@ static void launch_tile(const T* X, const T* A, const int32_t* map, const int32_
     constexpr int WPE = W4 ? 4 : 0;
     if (K == 27 && perm) {
         hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 27, LX, true, WPE>), grid, dim3(256), lds, s, X, A, map,
-                           perm, n_rows, M, n_mt, n_tiles, Y);
+                           perm, n_rows, M, n_mt, n_tiles, Y, bias, relu);
     } else if (K == 27) {
         hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 27, LX, false, WPE>), grid, dim3(256), lds, s, X, A, map,
-                           perm, n_rows, M, n_mt, n_tiles, Y);
+                           perm, n_rows, M, n_mt, n_tiles, Y, bias, relu);
     } else if (K == 8 && perm) {
         hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 8, LX, true, WPE>), grid, dim3(256), lds, s, X, A, map,
-                           perm, n_rows, M, n_mt, n_tiles, Y);
+                           perm, n_rows, M, n_mt, n_tiles, Y, bias, relu);
     } else if (K == 8) {
         hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 8, LX, false, WPE>), grid, dim3(256), lds, s, X, A, map,
-                           perm, n_rows, M, n_mt, n_tiles, Y);
+                           perm, n_rows, M, n_mt, n_tiles, Y, bias, relu);
     } else {
         hipLaunchKernelGGL((k_conv_tile<T, MI, NI, CIN, 1>), grid, dim3(256), lds, s, X, A, map, perm, n_rows, K, M,
-                           n_mt, n_tiles, Y);
+                           n_mt, n_tiles, Y, bias, relu);
     }
 }
 
@@ -561,20 +561,20 @@ constexpr size_t TILE_LDS_LIMIT = 65536;
 // fits (the best shape measured on MI355X), else 32 x 256 or 16 x 256 rows.
 template <typename T, int CIN>
 static bool try_tile(const T* X, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows, int K, int M,
-                     T* Y, hipStream_t s) {
+                     T* Y, const T* bias, int relu, hipStream_t s) {
     if constexpr (CIN == 64 && sizeof(T) == 2) {
         // 128-channel tiles for outputs in multiples of 128 channels and 27 taps
         if (M % 128 == 0 && K == 27) {
-            launch_tile<T, 8, 2, CIN>(X, A, map, perm, n_rows, K, M, Y, s);
+            launch_tile<T, 8, 2, CIN>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
             return true;
         }
     }
     if (M > 32 && tile_lds_bytes<T, 4, 2, CIN>(K) <= TILE_LDS_LIMIT) {
-        launch_tile<T, 4, 2, CIN>(X, A, map, perm, n_rows, K, M, Y, s);
+        launch_tile<T, 4, 2, CIN>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     } else if (M > 16 && tile_lds_bytes<T, 2, 4, CIN>(K) <= TILE_LDS_LIMIT) {
-        launch_tile<T, 2, 4, CIN>(X, A, map, perm, n_rows, K, M, Y, s);
+        launch_tile<T, 2, 4, CIN>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     } else if (tile_lds_bytes<T, 1, 4, CIN>(K) <= TILE_LDS_LIMIT) {
-        launch_tile<T, 1, 4, CIN>(X, A, map, perm, n_rows, K, M, Y, s);
+        launch_tile<T, 1, 4, CIN>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     } else {
         return false;
     }
@@ -585,7 +585,7 @@ static bool try_tile(const T* X, const T* A, const int32_t* map, const int32_t* 
 // occupancy chosen as for 64 input channels.
 template <typename T, int MI, int NI, int NCC>
 static void launch_tile_c(const T* X, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows, int K,
-                          int M, T* Y, hipStream_t s) {
+                          int M, T* Y, const T* bias, int relu, hipStream_t s) {
     constexpr int CC = 64;
     const int64_t n_rt = (n_rows + 64 * NI - 1) / (64 * NI);
     const int n_mt = (M + MI * 16 - 1) / (MI * 16);
@@ -602,10 +602,10 @@ static void launch_tile_c(const T* X, const T* A, const int32_t* map, const int3
     constexpr int LX8 = W4_8 ? 1 : (sizeof(T) == 2 ? 2 : 1);
     if (K == 27)
         hipLaunchKernelGGL((k_conv_tile_c<T, MI, NI, CC, NCC, 27, LX27, W4_27 ? 4 : 0>), grid, dim3(256), lds, s, X, A,
-                           map, perm, n_rows, M, n_mt, n_tiles, Y);
+                           map, perm, n_rows, M, n_mt, n_tiles, Y, bias, relu);
     else
         hipLaunchKernelGGL((k_conv_tile_c<T, MI, NI, CC, NCC, 8, LX8, W4_8 ? 4 : 0>), grid, dim3(256), lds, s, X, A,
-                           map, perm, n_rows, M, n_mt, n_tiles, Y);
+                           map, perm, n_rows, M, n_mt, n_tiles, Y, bias, relu);
 }
 
 // 16-bit outputs in multiples of 128 channels take 128-channel tiles (each gathered
@@ -617,23 +617,23 @@ static bool wide_m_tile(int M, int K, int ncc) {
 
 template <typename T, int NCC>
 static bool try_tile_c(const T* X, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows, int K, int M,
-                       T* Y, hipStream_t s) {
+                       T* Y, const T* bias, int relu, hipStream_t s) {
     if constexpr (sizeof(T) == 2) {
         if (wide_m_tile<T>(M, K, NCC)) {
-            launch_tile_c<T, 8, 2, NCC>(X, A, map, perm, n_rows, K, M, Y, s);
+            launch_tile_c<T, 8, 2, NCC>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
             return true;
         }
     }
-    if (M > 32) launch_tile_c<T, 4, 2, NCC>(X, A, map, perm, n_rows, K, M, Y, s);
-    else if (M > 16) launch_tile_c<T, 2, 4, NCC>(X, A, map, perm, n_rows, K, M, Y, s);
-    else launch_tile_c<T, 1, 4, NCC>(X, A, map, perm, n_rows, K, M, Y, s);
+    if (M > 32) launch_tile_c<T, 4, 2, NCC>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
+    else if (M > 16) launch_tile_c<T, 2, 4, NCC>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
+    else launch_tile_c<T, 1, 4, NCC>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     return true;
 }
 
 // Looped kernel only (runtime K), for wide inputs with other kernel volumes.
 template <typename T, int CIN>
 static bool try_tile_looped(const T* X, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows, int K,
-                            int M, T* Y, hipStream_t s) {
+                            int M, T* Y, const T* bias, int relu, hipStream_t s) {
     auto go = [&](auto mi, auto ni) {
         constexpr int MI = decltype(mi)::value, NI = decltype(ni)::value;
         const int64_t n_rt = (n_rows + 64 * NI - 1) / (64 * NI);
@@ -642,7 +642,7 @@ static bool try_tile_looped(const T* X, const T* A, const int32_t* map, const in
         const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
         const size_t lds = tile_lds_bytes<T, MI, NI, CIN>(K);
         hipLaunchKernelGGL((k_conv_tile<T, MI, NI, CIN, 1>), grid, dim3(256), lds, s, X, A, map, perm, n_rows, K, M,
-                           n_mt, n_tiles, Y);
+                           n_mt, n_tiles, Y, bias, relu);
     };
     if (M > 32 && tile_lds_bytes<T, 4, 2, CIN>(K) <= TILE_LDS_LIMIT) go(std::integral_constant<int, 4>{}, std::integral_constant<int, 2>{});
     else if (M > 16 && tile_lds_bytes<T, 2, 4, CIN>(K) <= TILE_LDS_LIMIT) go(std::integral_constant<int, 2>{}, std::integral_constant<int, 4>{});
@@ -654,17 +654,17 @@ static bool try_tile_looped(const T* X, const T* A, const int32_t* map, const in
 // LDS-tiled path for the channel counts it is compiled for; false if not applicable.
 template <typename T>
 inline bool conv_tile_dispatch_t(const T* X, int cin, const T* A, const int32_t* map, const int32_t* perm,
-                               int64_t n_rows, int K, int M, T* Y, hipStream_t s) {
+                               int64_t n_rows, int K, int M, T* Y, const T* bias, int relu, hipStream_t s) {
     if (K > TILE_MAX_K || ((uintptr_t)X % 16) != 0 || ((uintptr_t)A % 16) != 0) return false;
     switch (cin) {
-    case 32: return try_tile<T, 32>(X, A, map, perm, n_rows, K, M, Y, s);
-    case 64: return try_tile<T, 64>(X, A, map, perm, n_rows, K, M, Y, s);
+    case 32: return try_tile<T, 32>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
+    case 64: return try_tile<T, 64>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     case 128:
-        if (K == 27 || K == 8) return try_tile_c<T, 2>(X, A, map, perm, n_rows, K, M, Y, s);
-        return try_tile_looped<T, 128>(X, A, map, perm, n_rows, K, M, Y, s);
+        if (K == 27 || K == 8) return try_tile_c<T, 2>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
+        return try_tile_looped<T, 128>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     case 256:
-        if (K == 27 || K == 8) return try_tile_c<T, 4>(X, A, map, perm, n_rows, K, M, Y, s);
-        return try_tile_looped<T, 256>(X, A, map, perm, n_rows, K, M, Y, s);
+        if (K == 27 || K == 8) return try_tile_c<T, 4>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
+        return try_tile_looped<T, 256>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     default: return false;
     }
 }
@@ -672,10 +672,10 @@ inline bool conv_tile_dispatch_t(const T* X, int cin, const T* A, const int32_t*
 // Per element type (one translation unit each): launches the LDS-tiled kernel and
 // returns true, or returns false when the shape has no tiled instance.
 bool conv_tile_dispatch(const __bf16* X, int cin, const __bf16* A, const int32_t* map, const int32_t* perm,
-                        int64_t n_rows, int K, int M, __bf16* Y, hipStream_t s);
+                        int64_t n_rows, int K, int M, __bf16* Y, const __bf16* bias, int relu, hipStream_t s);
 bool conv_tile_dispatch(const _Float16* X, int cin, const _Float16* A, const int32_t* map, const int32_t* perm,
-                        int64_t n_rows, int K, int M, _Float16* Y, hipStream_t s);
+                        int64_t n_rows, int K, int M, _Float16* Y, const _Float16* bias, int relu, hipStream_t s);
 bool conv_tile_dispatch(const float* X, int cin, const float* A, const int32_t* map, const int32_t* perm,
-                        int64_t n_rows, int K, int M, float* Y, hipStream_t s);
+                        int64_t n_rows, int K, int M, float* Y, const float* bias, int relu, hipStream_t s);
 
 }  // namespace fvdbk

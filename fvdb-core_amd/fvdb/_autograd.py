@@ -11,6 +11,9 @@ mapped back with the inverse permutations (``:170-187``).
 * dgrad:   the same kernel over the opposite map with the un-transposed weights;
 * wgrad:   split-K MFMA reduction over the neighbour-map pairs with an fp32
   Neumaier-compensated sum of the chunk partials.
+* epilogue: an optional bias and ReLU applied to the fp32 accumulator before the
+  output is rounded (the module's separate bias add, ``fvdb/nn/modules.py:400-403``,
+  and a following ReLU, in one pass); the ReLU's gradient mask is read from the output.
 """
 from __future__ import annotations
 
@@ -61,27 +64,39 @@ def check_conv_args(feats: torch.Tensor, weight: torch.Tensor, n_in: int, transp
 
 
 class SparseConvFunction(torch.autograd.Function):
+    """out = act(conv(feats, weight) + bias) with the bias and ReLU fused into the
+    contraction's epilogue (one rounding to the feature dtype); bias may be None."""
+
     @staticmethod
-    def forward(ctx, feats, weight, pack, transposed: bool):
+    def forward(ctx, feats, weight, bias, pack, transposed: bool, relu: bool = False):
         n_src, n_tgt = pack.source_grid.total_voxels, pack.target_grid.total_voxels
         kio = weight_to_kio(weight, transposed)  # [K, I, O]
         if not transposed:
             fmap, order, n_out = pack._map_fwd(), pack._order_fwd(), n_tgt
         else:
             fmap, order, n_out = pack._map_bwd(), pack._order_bwd(), n_src
-        out = ext.implicit_gemm(feats.contiguous(), kio.transpose(1, 2).contiguous(), fmap, n_out, order)
-        ctx.save_for_backward(feats, kio)
+        out = ext.implicit_gemm(feats.contiguous(), kio.transpose(1, 2).contiguous(), fmap, n_out, order, bias,
+                                bool(relu))
+        ctx.save_for_backward(feats, kio, out if relu else None)
         ctx.pack = pack
         ctx.transposed = transposed
+        ctx.relu = bool(relu)
+        ctx.bias_dtype = bias.dtype if bias is not None else None
         return out
 
     @staticmethod
     def backward(ctx, grad_out):
-        feats, kio = ctx.saved_tensors
+        feats, kio, out = ctx.saved_tensors
         pack, transposed = ctx.pack, ctx.transposed
         grad_out = grad_out.contiguous()
         if grad_out.dtype != feats.dtype:
             grad_out = grad_out.to(feats.dtype)
+        if ctx.relu:
+            grad_out = torch.where(out > 0, grad_out, torch.zeros_like(grad_out))
+        gb = None
+        if ctx.needs_input_grad[2]:
+            gb = grad_out.sum(0, dtype=torch.float64 if grad_out.dtype == torch.float64 else torch.float32)
+            gb = gb.to(ctx.bias_dtype)
         gin = gw = None
         if ctx.needs_input_grad[0]:
             if not transposed:
@@ -93,10 +108,14 @@ class SparseConvFunction(torch.autograd.Function):
             pairs, sizes = pack._pairs()
             gk = ext.wgrad(feats.contiguous(), grad_out, pairs, sizes, transposed, wgrad_chunk(pairs.shape[0]))
             gw = kio_to_weight(gk.to(kio.dtype), pack.kernel_size, transposed)
-        return gin, gw, None, None
+        return gin, gw, gb, None, None, None
 
 
-def sparse_conv(feats: torch.Tensor, weight: torch.Tensor, pack, transposed: bool) -> torch.Tensor:
+def sparse_conv(feats: torch.Tensor, weight: torch.Tensor, pack, transposed: bool, bias=None,
+                relu: bool = False) -> torch.Tensor:
     n_in = pack.target_grid.total_voxels if transposed else pack.source_grid.total_voxels
     check_conv_args(feats, weight, n_in, transposed)
-    return SparseConvFunction.apply(feats, weight, pack, transposed)
+    cout = weight.shape[1] if transposed else weight.shape[0]
+    if bias is not None and (bias.dim() != 1 or bias.shape[0] != cout):
+        raise ValueError(f"bias must have shape [{cout}], got {tuple(bias.shape)}")
+    return SparseConvFunction.apply(feats, weight, bias, pack, transposed, relu)

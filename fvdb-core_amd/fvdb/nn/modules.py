@@ -195,7 +195,9 @@ class SparseConv3d(nn.Module):
             return ConvPackBackend.LGGS
         raise NotImplementedError(f"Backend {backend} is not supported")
 
-    def _dispatch_conv(self, in_feature: JaggedTensor, in_grid: GridBatch, in_kmap, out_grid):
+    def _dispatch_conv(self, in_feature: JaggedTensor, in_grid: GridBatch, in_kmap, out_grid, relu: bool = False):
+        """(out_grid, out_feature, out_kmap, fused): fused tells whether bias and ReLU
+        were applied in the convolution's epilogue."""
         backend = self.backend
         if backend == "cutlass" and (self.in_channels, self.out_channels) not in self.CUTLASS_SUPPORTED_CHANNELS:
             backend = "default"
@@ -207,13 +209,13 @@ class SparseConv3d(nn.Module):
             backend = "legacy" if in_feature.dtype == torch.float64 else "igemm_mode0"
         if backend == "halo" and self.stride == (1, 1, 1) and self.kernel_size == (3, 3, 3):
             assert out_grid is None or in_grid.is_same(out_grid)
-            return in_grid, in_grid.sparse_conv_halo(in_feature, self.weight, 8), None
+            return in_grid, in_grid.sparse_conv_halo(in_feature, self.weight, 8), None, False
         if backend == "dense" and self.stride == (1, 1, 1):
             assert out_grid is None or in_grid.is_same(out_grid)
             min_coord = in_grid.ijk.jdata.min(dim=0).values
             dense = in_grid.write_to_dense_czyx(in_feature, min_coord=min_coord)
             dense = torch.nn.functional.conv3d(dense, self.weight, padding=1, stride=1)
-            return in_grid, in_grid.read_from_dense_czyx(dense.contiguous(), dense_origins=min_coord), None
+            return in_grid, in_grid.read_from_dense_czyx(dense.contiguous(), dense_origins=min_coord), None, False
         if backend in ("halo", "dense"):
             backend = "igemm_mode0"
         can_cache = self.stride == (1, 1, 1) and (out_grid is None or out_grid.is_same(in_grid))
@@ -228,20 +230,27 @@ class SparseConv3d(nn.Module):
         out_kmap = kmap if can_cache else None
         be = self._build_kmap_and_convert_backend(kmap, backend)
         if not self.transposed:
-            out_feature = kmap.sparse_conv_3d(in_feature, self.weight, be)
+            out_feature = kmap.sparse_conv_3d(in_feature, self.weight, be, bias=self.bias, relu=relu)
         else:
-            out_feature = kmap.sparse_transpose_conv_3d(in_feature, self.weight, be)
-        return out_grid, out_feature, out_kmap
+            out_feature = kmap.sparse_transpose_conv_3d(in_feature, self.weight, be, bias=self.bias, relu=relu)
+        return out_grid, out_feature, out_kmap, True
 
-    def forward(self, input: VDBTensor, out_grid: Optional[GridBatch] = None) -> VDBTensor:
+    def forward(self, input: VDBTensor, out_grid: Optional[GridBatch] = None, *, fuse_relu: bool = False) -> VDBTensor:
+        """``fuse_relu`` applies a ReLU in the convolution's epilogue (same values as a
+        following ``fvdb.nn.ReLU``, without a separate pass over the output)."""
         in_feature, in_grid, in_kmap = input.data, input.grid, input.kmap
+        fused = False
         if self.kernel_size == (1, 1, 1) and self.stride == (1, 1, 1):
             out_feature = in_feature.jagged_like(in_feature.jdata.matmul(self.weight.transpose(0, 1)))
             out_grid, out_kmap = in_grid, in_kmap
         else:
-            out_grid, out_feature, out_kmap = self._dispatch_conv(in_feature, in_grid, in_kmap, out_grid)
-        if self.bias is not None:
-            out_feature.jdata = out_feature.jdata + self.bias
+            out_grid, out_feature, out_kmap, fused = self._dispatch_conv(in_feature, in_grid, in_kmap, out_grid,
+                                                                         fuse_relu)
+        if not fused:
+            if self.bias is not None:
+                out_feature.jdata = out_feature.jdata + self.bias
+            if fuse_relu:
+                out_feature.jdata = torch.relu(out_feature.jdata)
         if out_grid is None:
             raise ValueError("Failed to compute output grid. This is a bug in the implementation.")
         return VDBTensor(out_grid, out_feature, out_kmap)

@@ -9,7 +9,8 @@ Structure (levels l = 0..L-1, channels c_l, default 32/64/128/256):
   (2 c_l -> c_l -> c_l)
 * optional 1x1x1 head
 
-A ReLU follows every convolution except the head.  This is the usage pattern of the
+A ReLU follows every convolution except the head; it and the bias run in the
+convolution's epilogue (``SparseConv3d(..., fuse_relu=True)``).  This is the usage pattern of the
 reference's U-Net tutorial (``docs/tutorials/simple_unet.md:108-245``: strided down
 convs, ``transposed=True`` up convs with ``out_grid=``, ``jcat(dim=1)`` skips), without
 batch norm so that each voxel's output depends only on its own neighbourhood.
@@ -53,18 +54,18 @@ class SparseUNet(nn.Module):
         skips: List[VDBTensor] = []
         for l, convs in enumerate(self.enc):
             if l > 0:
-                x = self.relu(self.down[l - 1](x))
+                x = self.down[l - 1](x, fuse_relu=True)
             for conv in convs:
-                x = self.relu(conv(x))
+                x = conv(x, fuse_relu=True)
             skips.append(x)
         L = len(self.channels)
         for i, l in enumerate(range(L - 2, -1, -1)):
             skip = skips[l]
-            x = self.relu(self.up[i](x, out_grid=skip.grid))
+            x = self.up[i](x, out_grid=skip.grid, fuse_relu=True)
             x = fvdb.jcat([x, skip], dim=1)
             x = VDBTensor(x.grid, x.data, skip.kmap)  # same grid: reuse its stride-1 kernel map
             for conv in self.dec[i]:
-                x = self.relu(conv(x))
+                x = conv(x, fuse_relu=True)
         if self.head is not None:
             x = self.head(x)
         return x

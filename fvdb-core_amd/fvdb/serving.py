@@ -337,10 +337,7 @@ class UNetExecutor:
 
             def layer(inp, wb, m, n_rows):
                 a, b = wb
-                y = ext.implicit_gemm(inp, a, m, n_rows)
-                if b is not None:
-                    y = y + b
-                return torch.relu(y)
+                return ext.implicit_gemm(inp, a, m, n_rows, None, b, True)  # bias + ReLU in the epilogue
 
             skips = []
             for l in range(L):

@@ -245,25 +245,27 @@ class SparseConvPackInfo:
                 raise RuntimeError("LGGS buffer is not built")
 
     def sparse_conv_3d(self, input, weights: torch.Tensor,
-                       backend: ConvPackBackend = ConvPackBackend.GATHER_SCATTER) -> JaggedTensor:
+                       backend: ConvPackBackend = ConvPackBackend.GATHER_SCATTER, *, bias=None,
+                       relu: bool = False) -> JaggedTensor:
         j = input if isinstance(input, JaggedTensor) else JaggedTensor(input)
         if j.num_tensors != self._src.grid_count:
             raise ValueError("Input batch size must match target grid batch size")
         if j.element_count != self._src.total_voxels:
             raise ValueError("Input element count must match target grid total voxels")
         self._check_backend(backend, False)
-        out = sparse_conv(j.jdata, weights, self, False)
+        out = sparse_conv(j.jdata, weights, self, False, bias, relu)
         return self._tgt.jagged_like(out)
 
     def sparse_transpose_conv_3d(self, input, weights: torch.Tensor,
-                                 backend: ConvPackBackend = ConvPackBackend.GATHER_SCATTER) -> JaggedTensor:
+                                 backend: ConvPackBackend = ConvPackBackend.GATHER_SCATTER, *, bias=None,
+                                 relu: bool = False) -> JaggedTensor:
         j = input if isinstance(input, JaggedTensor) else JaggedTensor(input)
         if j.num_tensors != self._tgt.grid_count:
             raise ValueError("Input batch size must match target grid batch size")
         if j.element_count != self._tgt.total_voxels:
             raise ValueError("Input element count must match target grid total voxels")
         self._check_backend(backend, True)
-        out = sparse_conv(j.jdata, weights, self, True)
+        out = sparse_conv(j.jdata, weights, self, True, bias, relu)
         return self._src.jagged_like(out)
 
     # ----------------------------------------------------------------------------

@@ -241,6 +241,16 @@ int fvdbk_conv_implicit_gemm_sorted(fvdbk_dtype dtype, const void* in, int64_t n
                                     const int32_t* map, const int32_t* row_order, int64_t n_rows, int32_t K,
                                     int32_t M, void* out, uint32_t flags, hipStream_t stream);
 
+/* Contraction with a fused output epilogue: out = act(conv + bias), computed on the
+ * fp32 accumulator and rounded once to the output type.  bias is [M] in the feature
+ * dtype or NULL; flags: FVDBK_CONV_RELU applies max(., 0).  row_order may be NULL.
+ * Replaces the separate bias add and activation that follow a convolution in
+ * fvdb/nn/modules.py:400-403 (SparseConv3d bias) and the U-Net's ReLU modules. */
+#define FVDBK_CONV_RELU 1u
+int fvdbk_conv_implicit_gemm_fused(fvdbk_dtype dtype, const void* in, int64_t n_in, int32_t c_in, const void* a,
+                                   const int32_t* map, const int32_t* row_order, int64_t n_rows, int32_t K, int32_t M,
+                                   const void* bias, uint32_t flags, void* out, hipStream_t stream);
+
 /* keys[r] = bit k set when map[r][k] >= 0, for the first min(K, 31) taps (the
  * reference's bitmask of IGEMMBitOperations.cu:14-36 with one split). */
 int fvdbk_kmap_row_keys(const int32_t* map, int64_t n_rows, int32_t K, int32_t* keys, hipStream_t stream);

@@ -421,3 +421,52 @@ def test_windowed_row_order_bit_exact():
         assert torch.equal(fvdb.ext.implicit_gemm(x, a, kmap, n, o), ref)
     with pytest.raises(ValueError):
         fvdb.ext.row_order(kmap, n, -1)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("cin,cout", [(8, 12), (64, 64), (128, 128)])
+def test_fused_bias_relu_epilogue(dtype, cin, cout):
+    """act(conv + bias) in the contraction's epilogue equals conv, then + bias, then ReLU:
+    exactly in fp64 (forward and all gradients), within one extra rounding otherwise."""
+    fvdb = _fvdb()
+    z = golden("sphere_k3s1_c8.npz")
+    src = grids_from_fixture(fvdb, z)
+    pack, _ = src.sparse_conv_kernel_map(3, 1)
+    pack.build_implicit_gemm(training=True)
+    g = torch.Generator().manual_seed(cin * 7 + cout)
+    n = src.total_voxels
+    x = torch.randn(n, cin, generator=g, dtype=torch.float64).to("cuda", dtype)
+    w = (torch.randn(cout, cin, 3, 3, 3, generator=g, dtype=torch.float64) / math.sqrt(27 * cin)).to("cuda", dtype)
+    b = (0.3 * torch.randn(cout, generator=g, dtype=torch.float64)).to("cuda", dtype)
+    R = torch.randn(n, cout, generator=g, dtype=torch.float64).to("cuda", dtype)
+    outs = []
+    mask = None
+    for fused in (True, False):
+        xx, ww, bb = (t.clone().requires_grad_(True) for t in (x, w, b))
+        if fused:
+            y = pack.sparse_conv_3d(src.jagged_like(xx), ww, fvdb.ConvPackBackend.IGEMM, bias=bb, relu=True).jdata
+            mask = (y > 0).detach()
+        else:
+            pre = pack.sparse_conv_3d(src.jagged_like(xx), ww, fvdb.ConvPackBackend.IGEMM).jdata + bb
+            # fp64: the true composition; otherwise the fused output's ReLU mask, since a
+            # value within one rounding of zero may land on either side of the ReLU
+            y = torch.relu(pre) if dtype == torch.float64 else pre * mask
+        (y * R).sum().backward()
+        outs.append((y.detach(), xx.grad, ww.grad, bb.grad))
+    for name, a, r in zip(("out", "dgrad", "wgrad", "bgrad"), *outs):
+        if dtype == torch.float64:
+            assert torch.allclose(a, r, rtol=1e-12, atol=1e-12), name
+        else:
+            tol = 1e-5 if dtype == torch.float32 else 2e-2
+            if name == "out":
+                r = torch.relu(r)
+            assert rel_err(a, r) <= tol, f"{name}: {rel_err(a, r):.3e}"
+    # the module path: SparseConv3d(..., fuse_relu=True) == ReLU(SparseConv3d(...))
+    conv = fvdb.nn.SparseConv3d(cin, cout, 3).cuda().to(dtype)
+    v = fvdb.nn.VDBTensor(src, src.jagged_like(x))
+    a = conv(v, fuse_relu=True).data.jdata
+    r = fvdb.nn.ReLU()(conv(v)).data.jdata
+    if dtype == torch.float64:
+        assert torch.allclose(a, r, rtol=1e-12, atol=1e-12)
+    else:
+        assert rel_err(a, r) <= (1e-5 if dtype == torch.float32 else 2e-2)
