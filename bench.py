@@ -438,12 +438,15 @@ def run_diff(torch, fvdb, dev, args):
         x, w, R = x64.to(dt), w64.to(dt), R64.to(dt)
         kio = weight_to_kio(w, False)
         a_f = kio.transpose(1, 2).contiguous()
-        fwd = lambda: ext.implicit_gemm(x, a_f, pack._map_fwd(), N)  # noqa: E731
-        dgr = lambda: ext.implicit_gemm(R, kio, pack._map_bwd(), N)  # noqa: E731
+        # row orders as the autograd path picks them (tap-mask order per window, or tree order)
+        of, ob = pack._order_for(True, 128, dt), pack._order_for(False, 128, dt)
+        fwd = lambda: ext.implicit_gemm(x, a_f, pack._map_fwd(), N, of)  # noqa: E731
+        dgr = lambda: ext.implicit_gemm(R, kio, pack._map_bwd(), N, ob)  # noqa: E731
         wgr = lambda: ext.wgrad(x, R, pairs, sizes, False, chunk)  # noqa: E731
         t = {"fwd_ms": _median_ms(torch, fwd, 10), "dgrad_ms": _median_ms(torch, dgr, 10),
              "wgrad_ms": _median_ms(torch, wgr, 10)}
         t["tflops"] = {k[:-3]: F / (v * 1e-3) / 1e12 for k, v in list(t.items())}
+        t["row_order"] = "tap mask per 8192-row window" if of is not None else "tree"
         # autograd end to end: loss = sum(out * R)
         xr = x.clone().requires_grad_(True)
         wr = w.clone().requires_grad_(True)

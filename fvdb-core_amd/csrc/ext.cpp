@@ -314,17 +314,23 @@ at::Tensor row_order(at::Tensor map, int64_t n_rows, int64_t window) {
     require_hip(map, "map");
     TORCH_CHECK_VALUE(map.dim() == 2 && map.scalar_type() == at::kInt && map.size(0) >= n_rows,
                       "map must be int32 [>= rows, K]");
+    TORCH_CHECK_VALUE(window >= 0, "window must be >= 0");
     map = map.contiguous();
     c10::hip::HIPGuard guard(map.device().index());
+    if (window > 0) {
+        // per-window order, one kernel
+        at::Tensor order = at::empty({n_rows}, map.options());
+        check_rc(fvdbk_kmap_row_order_windowed(map.data_ptr<int32_t>(), n_rows, (int32_t)map.size(1), (int32_t)window,
+                                               order.data_ptr<int32_t>(), stream_of(map)),
+                 "kmap_row_order_windowed");
+        return order;
+    }
+    // one global order: stable descending sort of the 31-bit tap-mask keys
     at::Tensor keys = at::empty({n_rows}, map.options());
     check_rc(fvdbk_kmap_row_keys(map.data_ptr<int32_t>(), n_rows, (int32_t)map.size(1), keys.data_ptr<int32_t>(),
                                  stream_of(map)),
              "kmap_row_keys");
-    TORCH_CHECK_VALUE(window >= 0, "window must be >= 0");
-    at::Tensor comp = (int64_t(0x7fffffff) - keys.to(at::kLong));
-    if (window > 0 && window < n_rows)
-        comp = comp + at::floor_divide(at::arange(n_rows, comp.options()), window) * int64_t(0x80000000);
-    at::Tensor order = std::get<1>(at::sort(comp, /*stable=*/true, /*dim=*/0, /*descending=*/false));
+    at::Tensor order = std::get<1>(at::sort(keys, /*stable=*/true, /*dim=*/0, /*descending=*/true));
     return order.to(at::kInt).contiguous();
 }
 

@@ -204,18 +204,16 @@ __global__ __launch_bounds__(256) void k_conv_tile(const T* __restrict__ X, cons
 // Fully unrolled variant for a compile-time kernel volume KT (27 = 3^3, 8 = 2^3): the
 // pipeline registers (weights two taps ahead, gathered rows LX taps ahead) are indexed
 // by compile-time tap numbers, so the compiler keeps them in place.
-// SKIP (rows taken through a tap-mask ordering `perm`): taps that no row of the tile
-// uses are skipped, gathers and MFMAs; a skipped tap contributes exact zeros, so the
-// result is bit-identical to the unskipped form.  Without SKIP every tap is visited
-// (the tile of consecutive rows almost always uses all of them, and the branches cost
-// more than they save).
+// Rows are consecutive (tree order) and every tap is visited: a tile of neighbouring
+// rows almost always uses all of them.  Rows through a tap-mask ordering run the
+// compacted-step form of the wide-channel kernel below instead.
 // WPE > 0 asks the register allocator for WPE waves per SIMD (128 registers per
 // lane at 4): latency hiding, not registers, limits this kernel where it fits.
-template <typename T, int MI, int NI, int CIN, int KT, int LX, bool SKIP, int WPE>
+template <typename T, int MI, int NI, int CIN, int KT, int LX, int WPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, WPE > 0 ? WPE : 8)))
 void k_conv_tile_k(const T* __restrict__ X, const T* __restrict__ A,
                                                      const int32_t* __restrict__ map,
-                                                     const int32_t* __restrict__ perm, int64_t n_rows, int M, int n_mt,
+                                                     int64_t n_rows, int M, int n_mt,
                                                      int64_t n_tiles, T* __restrict__ Y, const T* __restrict__ bias, int relu) {
     using F = Frag<T>;
     using V = typename F::vec;
@@ -237,33 +235,14 @@ void k_conv_tile_k(const T* __restrict__ X, const T* __restrict__ A,
     const int64_t row0 = (t / n_mt) * RT;
     const int m0 = (int)(t % n_mt) * (MI * 16);
 
-    __shared__ unsigned int s_tmask;
-    if (SKIP) {
-        if (tid == 0) s_tmask = 0u;
-        __syncthreads();
-    }
-    // kernel-map tile -> LDS, and the taps that at least one row of the tile uses
-    unsigned int tm = 0u;
-    if (!SKIP) {
+    // kernel-map tile -> LDS
+    {
         const int64_t base = row0 * KT;
         const int64_t lim = n_rows * KT;
         for (int e = tid; e < RT * KT; e += 256) {
             const int32_t v = (base + e < lim) ? map[base + e] : -1;
             sMap[e] = v;
         }
-    } else {
-        for (int e = tid; e < RT * KT; e += 256) {
-            const int i = e / KT, k = e - i * KT;
-            const int64_t r = row0 + i;
-            const int32_t v = r < n_rows ? map[(int64_t)perm[r] * KT + k] : -1;
-            sMap[e] = v;
-            if (v >= 0) tm |= 1u << k;
-        }
-    }
-    if (SKIP) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) tm |= __shfl_xor(tm, off);
-        if (lane == 0 && tm) atomicOr(&s_tmask, tm);
     }
 
     floatx4 acc[MI][NI];
@@ -310,23 +289,19 @@ void k_conv_tile_k(const T* __restrict__ X, const T* __restrict__ A,
         }
     };
 
-    __syncthreads();  // map tile and tap mask visible
-    // Taps no row of the tile uses are skipped (gathers and MFMAs; the weight pipeline
-    // keeps its fixed schedule).  Rows ordered by their tap mask make this frequent.
-    const unsigned int tmask = SKIP ? s_tmask : ~0u;
+    __syncthreads();  // map tile visible
     fetch(0, w[0]);
     store(0, w[0]);
     if (KT > 1) fetch(1, w[1]);
 #pragma unroll
-    for (int k = 0; k < LX && k < KT; ++k)
-        if (tmask & (1u << k)) gather(k, b[k]);
+    for (int k = 0; k < LX && k < KT; ++k) gather(k, b[k]);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < KT; ++k) {
         // weights two taps ahead, input rows LX taps ahead
         if (k + 2 < KT) fetch(k + 2, w[(k + 2) % 3]);
-        if (k + LX < KT && (tmask & (1u << (k + LX)))) gather(k + LX, b[(k + LX) % (LX + 1)]);
-        if (tmask & (1u << k)) {
+        if (k + LX < KT) gather(k + LX, b[(k + LX) % (LX + 1)]);
+        {
             const T* wb = sW + (k % 2) * WT + r16 * WP + g * F::EPL;
 #pragma unroll
             for (int ch = 0; ch < NCH; ++ch) {
@@ -348,8 +323,7 @@ void k_conv_tile_k(const T* __restrict__ X, const T* __restrict__ A,
     for (int j = 0; j < NI; ++j) {
         const int64_t ri = row0 + wave * NI * 16 + j * 16 + r16;
         if (ri >= n_rows) continue;
-        const int64_t r = SKIP ? (int64_t)perm[ri] : ri;
-        T* __restrict__ y = Y + r * M;
+        T* __restrict__ y = Y + ri * M;
 #pragma unroll
         for (int i = 0; i < MI; ++i) {
 #pragma unroll
@@ -367,10 +341,14 @@ void k_conv_tile_k(const T* __restrict__ X, const T* __restrict__ A,
 // step) however wide the input is.  Up to 64 steps are fully unrolled; longer
 // sequences run a loop whose body is 6 steps unrolled (a multiple of the 3-entry weight
 // ring, the 2-entry LDS ring and the LX+1-entry gather ring), which bounds the code
-// size.  Rows may come through a permutation `perm` (no tap skipping here).  The
-// accumulation order per row is taps ascending, then chunks ascending, then MFMA
-// chunks: fixed, so results do not depend on the launch.
-template <typename T, int MI, int NI, int CC, int NCC, int KT, int LX, int WPE>
+// size.  The accumulation order per row is taps ascending, then chunks ascending,
+// then MFMA chunks: fixed, so results do not depend on the launch.
+// SKIP (rows taken through a tap-mask ordering `perm`): the steps run over the taps
+// that at least one row of the tile uses, listed in LDS at the start, so a skipped tap
+// costs neither its gathers, its weight fetch, its barrier nor its MFMAs.  A skipped tap
+// would have added exact zeros, so the result is bit-identical to the full sequence
+// for finite weights.
+template <typename T, int MI, int NI, int CC, int NCC, int KT, int LX, int WPE, bool SKIP = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, WPE > 0 ? WPE : 8)))
 void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32_t* __restrict__ map,
                    const int32_t* __restrict__ perm, int64_t n_rows, int M, int n_mt, int64_t n_tiles,
@@ -399,17 +377,51 @@ void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32
     const int64_t row0 = (t / n_mt) * RT;
     const int m0 = (int)(t % n_mt) * (MI * 16);
 
+    static_assert(!SKIP || KT <= 32, "tap masks are 32 bits");
+    static_assert((NCC & (NCC - 1)) == 0, "chunk count must be a power of two");
+    __shared__ unsigned int s_tmask;
+    if (SKIP) {
+        if (tid == 0) s_tmask = 0u;
+        __syncthreads();
+    }
     if (perm == nullptr) {
         const int64_t base = row0 * KT;
         const int64_t lim = n_rows * KT;
         for (int e = tid; e < RT * KT; e += 256) sMap[e] = (base + e < lim) ? map[base + e] : -1;
     } else {
+        unsigned int tm = 0u;
         for (int e = tid; e < RT * KT; e += 256) {
             const int i = e / KT, k = e - i * KT;
             const int64_t r = row0 + i;
-            sMap[e] = r < n_rows ? map[(int64_t)perm[r] * KT + k] : -1;
+            const int32_t v = r < n_rows ? map[(int64_t)perm[r] * KT + k] : -1;
+            sMap[e] = v;
+            if (SKIP && v >= 0) tm |= 1u << k;
+        }
+        if (SKIP) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) tm |= __shfl_xor(tm, off);
+            if (lane == 0 && tm) atomicOr(&s_tmask, tm);
         }
     }
+    __syncthreads();  // map tile and tap mask visible
+    // Steps: all (tap, chunk) pairs, or those of the taps the tile uses.  The fetch and
+    // gather streams each walk the tile's taps with a uniform (scalar) bit iterator: the
+    // lowest set bit is the tap of the stream's next step, cleared after its last chunk.
+    unsigned int fit = 0u, git = 0u;
+    int kv = KV;
+    if (SKIP) {
+        fit = git = (unsigned int)__builtin_amdgcn_readfirstlane((int)s_tmask);
+        kv = __popc(fit) * NCC;
+    }
+    auto next_tap = [&](unsigned int& it, int v) -> int {
+        if constexpr (SKIP) {
+            const int tap = __builtin_ctz(it);
+            if ((v & (NCC - 1)) == NCC - 1) it &= it - 1u;
+            return tap;
+        } else {
+            return v / NCC;
+        }
+    };
 
     floatx4 acc[MI][NI];
 #pragma unroll
@@ -419,8 +431,9 @@ void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32
 
     V w[3][SEG_PER_T];
     V b[LX + 1][NI][NCH];
+    // weights of step v (called once per step, in step order)
     auto fetch = [&](int v, V (&wr)[SEG_PER_T]) {
-        const int tap = v / NCC, c0 = (v - tap * NCC) * CC;
+        const int tap = next_tap(fit, v), c0 = (v & (NCC - 1)) * CC;
 #pragma unroll
         for (int q = 0; q < SEG_PER_T; ++q) {
             const int sgi = tid + q * 256;
@@ -441,8 +454,9 @@ void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32
             }
         }
     };
+    // gathered input rows of step v (called once per step, in step order)
     auto gather = [&](int v, V (&bb)[NI][NCH]) {
-        const int tap = v / NCC, c0 = (v - tap * NCC) * CC;
+        const int tap = next_tap(git, v), c0 = (v & (NCC - 1)) * CC;
 #pragma unroll
         for (int j = 0; j < NI; ++j) {
             const int32_t src = sMap[(wave * NI * 16 + j * 16 + r16) * KT + tap];
@@ -458,8 +472,8 @@ void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32
     // one step; u = v mod U selects the ring entries at compile time
     auto step = [&](int v, auto uc) {
         constexpr int u = decltype(uc)::value;
-        if (v + 2 < KV) fetch(v + 2, w[(u + 2) % 3]);
-        if (v + LX < KV) gather(v + LX, b[(u + LX) % (LX + 1)]);
+        if (v + 2 < kv) fetch(v + 2, w[(u + 2) % 3]);
+        if (v + LX < kv) gather(v + LX, b[(u + LX) % (LX + 1)]);
         const T* wb = sW + (u % 2) * WT + r16 * WP + g * F::EPL;
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) {
@@ -470,33 +484,38 @@ void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32
                 for (int j = 0; j < NI; ++j) mfma(a, b[u % (LX + 1)][j][ch], acc[i][j]);
             }
         }
-        if (v + 1 < KV) {
+        if (v + 1 < kv) {
             store((u + 1) % 2, w[(u + 1) % 3]);
             __syncthreads();
         }
     };
 
-    __syncthreads();  // map tile visible
-    fetch(0, w[0]);
-    store(0, w[0]);
-    if (KV > 1) fetch(1, w[1]);
+    if (kv > 0) {
+        fetch(0, w[0]);
+        store(0, w[0]);
+    }
+    if (kv > 1) fetch(1, w[1]);
 #pragma unroll
-    for (int v = 0; v < LX && v < KV; ++v) gather(v, b[v]);
+    for (int v = 0; v < LX && v < KV; ++v)
+        if (v < kv) gather(v, b[v]);
     __syncthreads();
+    // a step count known at compile time lets whole six-step groups drop their checks
+    constexpr bool FULL = !SKIP && KV % U == 0;
     auto six = [&](int v0) {
         step(v0 + 0, std::integral_constant<int, 0>{});
-        if (KV % U == 0 || v0 + 1 < KV) step(v0 + 1, std::integral_constant<int, 1>{});
-        if (KV % U == 0 || v0 + 2 < KV) step(v0 + 2, std::integral_constant<int, 2>{});
-        if (KV % U == 0 || v0 + 3 < KV) step(v0 + 3, std::integral_constant<int, 3>{});
-        if (KV % U == 0 || v0 + 4 < KV) step(v0 + 4, std::integral_constant<int, 4>{});
-        if (KV % U == 0 || v0 + 5 < KV) step(v0 + 5, std::integral_constant<int, 5>{});
+        if (FULL || v0 + 1 < kv) step(v0 + 1, std::integral_constant<int, 1>{});
+        if (FULL || v0 + 2 < kv) step(v0 + 2, std::integral_constant<int, 2>{});
+        if (FULL || v0 + 3 < kv) step(v0 + 3, std::integral_constant<int, 3>{});
+        if (FULL || v0 + 4 < kv) step(v0 + 4, std::integral_constant<int, 4>{});
+        if (FULL || v0 + 5 < kv) step(v0 + 5, std::integral_constant<int, 5>{});
     };
-    if constexpr (KV <= 64) {
+    if constexpr (!SKIP && KV <= 64) {
+        // fully unrolled: step numbers and taps are compile-time
 #pragma unroll
         for (int v0 = 0; v0 < KV; v0 += U) six(v0);
     } else {
 #pragma unroll 1
-        for (int v0 = 0; v0 < KV; v0 += U) six(v0);
+        for (int v0 = 0; v0 < kv; v0 += U) six(v0);
     }
 
 #pragma unroll
@@ -537,18 +556,20 @@ static void launch_tile(const T* X, const T* A, const int32_t* map, const int32_
     constexpr bool W4 = sizeof(T) == 2 && MI <= 4 && (CIN <= 32 || (CIN == 64 && NI == 2) || (CIN == 64 && MI == 1));
     constexpr int LX = W4 ? 1 : (sizeof(T) == 2 ? 2 : 1);
     constexpr int WPE = W4 ? 4 : 0;
+    // rows through a tap-mask ordering: the compacted-step kernel (one chunk per tap),
+    // which skips the weights, barriers, gathers and MFMAs of the taps a tile does not use
     if (K == 27 && perm) {
-        hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 27, LX, true, WPE>), grid, dim3(256), lds, s, X, A, map,
+        hipLaunchKernelGGL((k_conv_tile_c<T, MI, NI, CIN, 1, 27, LX, WPE, true>), grid, dim3(256), lds, s, X, A, map,
                            perm, n_rows, M, n_mt, n_tiles, Y, bias, relu);
     } else if (K == 27) {
-        hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 27, LX, false, WPE>), grid, dim3(256), lds, s, X, A, map,
-                           perm, n_rows, M, n_mt, n_tiles, Y, bias, relu);
+        hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 27, LX, WPE>), grid, dim3(256), lds, s, X, A, map, n_rows,
+                           M, n_mt, n_tiles, Y, bias, relu);
     } else if (K == 8 && perm) {
-        hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 8, LX, true, WPE>), grid, dim3(256), lds, s, X, A, map,
+        hipLaunchKernelGGL((k_conv_tile_c<T, MI, NI, CIN, 1, 8, LX, WPE, true>), grid, dim3(256), lds, s, X, A, map,
                            perm, n_rows, M, n_mt, n_tiles, Y, bias, relu);
     } else if (K == 8) {
-        hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 8, LX, false, WPE>), grid, dim3(256), lds, s, X, A, map,
-                           perm, n_rows, M, n_mt, n_tiles, Y, bias, relu);
+        hipLaunchKernelGGL((k_conv_tile_k<T, MI, NI, CIN, 8, LX, WPE>), grid, dim3(256), lds, s, X, A, map, n_rows,
+                           M, n_mt, n_tiles, Y, bias, relu);
     } else {
         hipLaunchKernelGGL((k_conv_tile<T, MI, NI, CIN, 1>), grid, dim3(256), lds, s, X, A, map, perm, n_rows, K, M,
                            n_mt, n_tiles, Y, bias, relu);
@@ -600,9 +621,16 @@ static void launch_tile_c(const T* X, const T* A, const int32_t* map, const int3
     constexpr bool W4_8 = sizeof(T) == 2 && MI <= 4 && (NI == 2 || MI == 1);
     constexpr int LX27 = W4_27 ? 1 : (sizeof(T) == 2 ? 2 : 1);
     constexpr int LX8 = W4_8 ? 1 : (sizeof(T) == 2 ? 2 : 1);
-    if (K == 27)
+    // rows through a tap-mask ordering skip the taps a tile does not use
+    if (K == 27 && perm)
+        hipLaunchKernelGGL((k_conv_tile_c<T, MI, NI, CC, NCC, 27, LX27, W4_27 ? 4 : 0, true>), grid, dim3(256), lds, s,
+                           X, A, map, perm, n_rows, M, n_mt, n_tiles, Y, bias, relu);
+    else if (K == 27)
         hipLaunchKernelGGL((k_conv_tile_c<T, MI, NI, CC, NCC, 27, LX27, W4_27 ? 4 : 0>), grid, dim3(256), lds, s, X, A,
                            map, perm, n_rows, M, n_mt, n_tiles, Y, bias, relu);
+    else if (perm)
+        hipLaunchKernelGGL((k_conv_tile_c<T, MI, NI, CC, NCC, 8, LX8, W4_8 ? 4 : 0, true>), grid, dim3(256), lds, s, X,
+                           A, map, perm, n_rows, M, n_mt, n_tiles, Y, bias, relu);
     else
         hipLaunchKernelGGL((k_conv_tile_c<T, MI, NI, CC, NCC, 8, LX8, W4_8 ? 4 : 0>), grid, dim3(256), lds, s, X, A,
                            map, perm, n_rows, M, n_mt, n_tiles, Y, bias, relu);

@@ -8,6 +8,8 @@
 
 #include <string>
 
+#include <hipcub/block/block_radix_sort.hpp>
+
 #include "fvdbk_common.h"
 
 namespace fvdbk {
@@ -570,6 +572,77 @@ __global__ __launch_bounds__(NB_THREADS) void k_nbmap_emit(const int32_t* __rest
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Row order by tap mask within windows of consecutive rows.  One workgroup owns
+// ORDER_BLOCK consecutive rows: it builds each row's key (window index within the
+// block above the inverted tap mask of the first min(K, 27) taps, so windows never mix
+// and masks sort descending), runs a block-wide radix sort over just those key bits
+// (stable: ties keep row order), and writes the rows in sorted order.  Rows past
+// n_rows take the largest key and, being last in the block, stay behind every real row.
+// ---------------------------------------------------------------------------
+constexpr int ORDER_THREADS = 1024;
+constexpr int ORDER_ITEMS = 8;
+constexpr int ORDER_BLOCK = ORDER_THREADS * ORDER_ITEMS;
+constexpr int ORDER_LOG2_BLOCK = 13;
+constexpr int ORDER_MASK_BITS = 27;
+static_assert(ORDER_BLOCK == 1 << ORDER_LOG2_BLOCK, "block size");
+
+// KT > 0: compile-time kernel volume (the tap loads of a row are unrolled and issued
+// together); KT = 0: runtime K.
+template <int KT>
+__global__ __launch_bounds__(ORDER_THREADS) void k_row_order_window(const int32_t* __restrict__ map, int64_t n_rows,
+                                                                    int K_rt, int log2_window,
+                                                                    int32_t* __restrict__ order) {
+    using Sort = hipcub::BlockRadixSort<uint32_t, ORDER_THREADS, ORDER_ITEMS, int32_t>;
+    __shared__ union {
+        typename Sort::TempStorage sort;
+        uint32_t keys[ORDER_BLOCK];
+    } sm;
+    const int K = KT > 0 ? KT : K_rt;
+    const int kb = K < ORDER_MASK_BITS ? K : ORDER_MASK_BITS;
+    const int sub_bits = ORDER_LOG2_BLOCK - log2_window;
+    const uint32_t full = (1u << kb) - 1u;
+    const int64_t base = (int64_t)blockIdx.x * ORDER_BLOCK;
+    // keys, one row per lane (neighbouring lanes read neighbouring map rows)
+#pragma unroll
+    for (int it = 0; it < ORDER_ITEMS; ++it) {
+        const int local = threadIdx.x + it * ORDER_THREADS;
+        const int64_t r = base + local;
+        uint32_t key = (sub_bits + kb == 32) ? 0xFFFFFFFFu : ((1u << (sub_bits + kb)) - 1u);
+        if (r < n_rows) {
+            const int32_t* m = map + r * K;
+            uint32_t mask = 0u;
+            if constexpr (KT > 0) {
+#pragma unroll
+                for (int k = 0; k < (KT < ORDER_MASK_BITS ? KT : ORDER_MASK_BITS); ++k)
+                    mask |= (m[k] >= 0 ? 1u : 0u) << k;
+            } else {
+                for (int k = 0; k < kb; ++k) mask |= (m[k] >= 0 ? 1u : 0u) << k;
+            }
+            key = ((uint32_t)(local >> log2_window) << kb) | (full - mask);
+        }
+        sm.keys[local] = key;
+    }
+    __syncthreads();
+    // blocked arrangement: lane t holds rows t*ITEMS .. t*ITEMS+ITEMS-1
+    uint32_t key[ORDER_ITEMS];
+    int32_t row[ORDER_ITEMS];
+#pragma unroll
+    for (int i = 0; i < ORDER_ITEMS; ++i) {
+        const int local = threadIdx.x * ORDER_ITEMS + i;
+        key[i] = sm.keys[local];
+        row[i] = (int32_t)(base + local);
+    }
+    __syncthreads();  // the key array and the sort's storage share memory
+    Sort(sm.sort).Sort(key, row, 0, sub_bits + kb);
+#pragma unroll
+    for (int i = 0; i < ORDER_ITEMS; ++i) {
+        const int64_t pos = base + threadIdx.x * ORDER_ITEMS + i;
+        if (pos < n_rows) order[pos] = row[i];
+    }
+}
+
 }  // namespace fvdbk
 
 using namespace fvdbk;
@@ -799,6 +872,26 @@ int fvdbk_kmap_build(const fvdbk_grid* rows, const fvdbk_grid* probe, const int3
     }
     g.sgn = inverse ? -1 : 1;
     return kmap_launch(rows, probe, g, map, n_rows_padded, stream);
+}
+
+int fvdbk_kmap_row_order_windowed(const int32_t* map, int64_t n_rows, int32_t K, int32_t window, int32_t* order,
+                                  hipStream_t stream) {
+    FVDBK_REQUIRE(K > 0, FVDBK_EINVAL, "kernel volume must be positive");
+    FVDBK_REQUIRE(window >= 256 && window <= ORDER_BLOCK && (window & (window - 1)) == 0, FVDBK_EINVAL,
+                  "window must be a power of two in [256, %d], got %d", ORDER_BLOCK, (int)window);
+    FVDBK_REQUIRE(n_rows >= 0 && n_rows < ((int64_t)1 << 31), FVDBK_EINVAL, "row count out of range");
+    if (n_rows == 0) return FVDBK_OK;
+    const int log2w = __builtin_ctz((unsigned)window);
+    const int64_t blocks = (n_rows + ORDER_BLOCK - 1) / ORDER_BLOCK;
+    const dim3 grid((unsigned)blocks), block(ORDER_THREADS);
+    if (K == 27)
+        hipLaunchKernelGGL(k_row_order_window<27>, grid, block, 0, stream, map, n_rows, (int)K, log2w, order);
+    else if (K == 8)
+        hipLaunchKernelGGL(k_row_order_window<8>, grid, block, 0, stream, map, n_rows, (int)K, log2w, order);
+    else
+        hipLaunchKernelGGL(k_row_order_window<0>, grid, block, 0, stream, map, n_rows, (int)K, log2w, order);
+    FVDBK_LAUNCH_CHECK("k_row_order_window");
+    return FVDBK_OK;
 }
 
 int fvdbk_kmap_window(const fvdbk_grid* rows, const fvdbk_grid* probe, const int32_t window[3],

@@ -11,6 +11,8 @@ mapped back with the inverse permutations (``:170-187``).
 * dgrad:   the same kernel over the opposite map with the un-transposed weights;
 * wgrad:   split-K MFMA reduction over the neighbour-map pairs with an fp32
   Neumaier-compensated sum of the chunk partials.
+* row order: rows grouped by tap mask (the pack's ``_order_for``) so that a tile skips
+  the taps none of its rows uses; the result does not depend on the order.
 * epilogue: an optional bias and ReLU applied to the fp32 accumulator before the
   output is rounded (the module's separate bias add, ``fvdb/nn/modules.py:400-403``,
   and a following ReLU, in one pass); the ReLU's gradient mask is read from the output.
@@ -71,10 +73,11 @@ class SparseConvFunction(torch.autograd.Function):
     def forward(ctx, feats, weight, bias, pack, transposed: bool, relu: bool = False):
         n_src, n_tgt = pack.source_grid.total_voxels, pack.target_grid.total_voxels
         kio = weight_to_kio(weight, transposed)  # [K, I, O]
+        cin, dt = feats.shape[1], feats.dtype
         if not transposed:
-            fmap, order, n_out = pack._map_fwd(), pack._order_fwd(), n_tgt
+            fmap, order, n_out = pack._map_fwd(), pack._order_for(True, cin, dt), n_tgt
         else:
-            fmap, order, n_out = pack._map_bwd(), pack._order_bwd(), n_src
+            fmap, order, n_out = pack._map_bwd(), pack._order_for(False, cin, dt), n_src
         out = ext.implicit_gemm(feats.contiguous(), kio.transpose(1, 2).contiguous(), fmap, n_out, order, bias,
                                 bool(relu))
         ctx.save_for_backward(feats, kio, out if relu else None)
@@ -99,10 +102,11 @@ class SparseConvFunction(torch.autograd.Function):
             gb = gb.to(ctx.bias_dtype)
         gin = gw = None
         if ctx.needs_input_grad[0]:
+            cg, dt = grad_out.shape[1], grad_out.dtype
             if not transposed:
-                dmap, order, n_in = pack._map_bwd(), pack._order_bwd(), pack.source_grid.total_voxels
+                dmap, order, n_in = pack._map_bwd(), pack._order_for(False, cg, dt), pack.source_grid.total_voxels
             else:
-                dmap, order, n_in = pack._map_fwd(), pack._order_fwd(), pack.target_grid.total_voxels
+                dmap, order, n_in = pack._map_fwd(), pack._order_for(True, cg, dt), pack.target_grid.total_voxels
             gin = ext.implicit_gemm(grad_out, kio, dmap, n_in, order)
         if ctx.needs_input_grad[1]:
             pairs, sizes = pack._pairs()

@@ -28,6 +28,8 @@ from typing import Any, Callable, Dict, List, Optional, Sequence
 
 import torch
 
+from .sparse_conv_pack_info import AUTO_ORDER_WINDOW, wants_row_order
+
 __all__ = ["Request", "ContinuousBatcher", "UNetExecutor", "UNetTopology", "latency_stats"]
 
 
@@ -335,9 +337,17 @@ class UNetExecutor:
             up = [_shift_cat([t.up[l] for t in topos], offs[l + 1][:-1]) for l in range(L - 1)]
             x = torch.cat([r.state["x"] for r in reqs], 0) if len(reqs) > 1 else reqs[0].state["x"]
 
-            def layer(inp, wb, m, n_rows):
+            orders = {}
+
+            def layer(inp, wb, m, n_rows, one_tap=False):
                 a, b = wb
-                return ext.implicit_gemm(inp, a, m, n_rows, None, b, True)  # bias + ReLU in the epilogue
+                # rows grouped by tap mask per window where that pays (same rule as the packs)
+                order = None
+                if n_rows > 0 and wants_row_order(m.shape[1], inp.shape[1], inp.dtype, one_tap):
+                    if id(m) not in orders:
+                        orders[id(m)] = ext.row_order(m, n_rows, AUTO_ORDER_WINDOW)
+                    order = orders[id(m)]
+                return ext.implicit_gemm(inp, a, m, n_rows, order, b, True)  # bias + ReLU in the epilogue
 
             skips = []
             for l in range(L):
@@ -349,7 +359,7 @@ class UNetExecutor:
                 skips.append(x)
             for i, l in enumerate(range(L - 2, -1, -1)):
                 n_l = offs[l][-1]
-                x = layer(x, self.up[i], up[l], n_l)
+                x = layer(x, self.up[i], up[l], n_l, one_tap=True)
                 x = torch.cat([x, skips[l]], 1)
                 for wb in self.dec[i]:
                     x = layer(x, wb, conv[l], n_l)

@@ -255,6 +255,14 @@ int fvdbk_conv_implicit_gemm_fused(fvdbk_dtype dtype, const void* in, int64_t n_
  * reference's bitmask of IGEMMBitOperations.cu:14-36 with one split). */
 int fvdbk_kmap_row_keys(const int32_t* map, int64_t n_rows, int32_t K, int32_t* keys, hipStream_t stream);
 
+/* Row order for fvdbk_conv_implicit_gemm_sorted that keeps locality: each window of
+ * `window` consecutive rows (a power of two in [256, 8192]) is ordered by tap mask,
+ * descending over the first min(K, 27) taps, ties in row order; windows stay in
+ * place.  The windowed form of the reference's reorder_loc (SparseConvPackInfo.cpp:117-128,
+ * one global sort): rows of a tile share most taps while their neighbours stay close. */
+int fvdbk_kmap_row_order_windowed(const int32_t* map, int64_t n_rows, int32_t K, int32_t window, int32_t* order,
+                                  hipStream_t stream);
+
 /* ---------------------------------------------------------------------------
  * Weight gradient (replaces dispatchSparseConvolutionImplicitGEMMGrad{,Sorted},
  * backend/SparseConvolutionImplicitGEMMGrad.h, and the kernel-gradient half of

@@ -30,6 +30,8 @@ Parity pins (each checked in ``tests/test_oracle.py``):
   ``torch.nn.functional.max_pool3d`` / ``avg_pool3d`` / ``interpolate(mode="nearest")``
   on dense tensors that hold -inf / 0 / nothing at missing voxels, on dense boxes and
   random sparse sets, and set intersection for injection.
+* row order by tap mask (global, and per window of consecutive rows): a brute-force
+  ``sorted`` with an explicit (window, -mask, row) key on small random maps.
 """
 from __future__ import annotations
 
@@ -369,3 +371,21 @@ def inject_index(dst_grids: Sequence[np.ndarray], src_grids: Sequence[np.ndarray
         for c in np.asarray(g).reshape(-1, 3).tolist():
             out.append(src.get((b, c[0], c[1], c[2]), -1))
     return np.asarray(out, dtype=np.int64)
+
+
+def row_order(kmap: np.ndarray, window: int = 0) -> np.ndarray:
+    """Rows ordered by tap mask, descending, ties in row order; with ``window`` > 0 each
+    window of that many consecutive rows is ordered on its own and stays in place.
+
+    The global form is the reference's ``reorder_loc`` (``SparseConvPackInfo.cpp:117-128``:
+    a descending sort of the per-row bitmask of ``IGEMMBitOperations.cu:14-36``), made
+    deterministic by keeping ties in row order.  The bitmask covers the first
+    min(K, 31) taps (min(K, 27) in the windowed form).
+    """
+    kmap = np.asarray(kmap)
+    n, K = kmap.shape
+    bits = min(K, 27 if window > 0 else 31)
+    mask = ((kmap[:, :bits] >= 0).astype(np.int64) << np.arange(bits, dtype=np.int64)).sum(1)
+    rows = np.arange(n, dtype=np.int64)
+    win = rows // window if window > 0 else np.zeros(n, np.int64)
+    return np.lexsort((rows, -mask, win)).astype(np.int32)

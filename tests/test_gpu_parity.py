@@ -400,27 +400,79 @@ def test_sorted_rows_bit_exact(dtype, cin, cout):
                 assert torch.equal(a, b)
 
 
-def test_windowed_row_order_bit_exact():
-    """A row order sorted within windows of consecutive rows is a permutation and leaves the result unchanged."""
+def test_row_orders_match_oracle():
+    """Global and windowed row orders equal the oracle's (a deterministic sort: descending
+    tap mask, ties in row order, windows in place), on one block and on many, and the
+    contraction through them equals the unordered one bit for bit."""
     fvdb = _fvdb()
+    from fvdb.utils.synthetic import sphere_points
+
     z = golden("sphere_k3s1_c8.npz")
-    src = grids_from_fixture(fvdb, z)
-    pack, _ = src.sparse_conv_kernel_map(3, 1)
-    pack.build_implicit_gemm()
-    n = src.total_voxels
-    kmap = pack._map_fwd()
+    small = grids_from_fixture(fvdb, z)
+    big = fvdb.GridBatch.from_points(sphere_points(seed=4, n_points=200_000, device="cuda"), voxel_sizes=1 / 160)
     g = torch.Generator().manual_seed(3)
-    x = torch.randn(n, 64, generator=g).to("cuda", torch.bfloat16)
-    a = (0.1 * torch.randn(27, 64, 64, generator=g)).to("cuda", torch.bfloat16)
-    ref = fvdb.ext.implicit_gemm(x, a, kmap, n)
-    for win in (1, 100, 512, n + 5):
-        o = fvdb.ext.row_order(kmap, n, win)
-        ol = o.long().cpu()
-        assert torch.equal(torch.sort(ol).values, torch.arange(n))
-        assert bool(((ol // win) == (torch.arange(n) // win)).all())  # stays inside its window
-        assert torch.equal(fvdb.ext.implicit_gemm(x, a, kmap, n, o), ref)
-    with pytest.raises(ValueError):
-        fvdb.ext.row_order(kmap, n, -1)
+    for grid in (small, big):
+        for ks, st in ((3, 1), (2, 2)):
+            pack, tgt = grid.sparse_conv_kernel_map(ks, st)
+            pack.build_implicit_gemm(training=True)
+            # (map, rows, voxels its entries index)
+            for kmap, n, n_in in ((pack._map_fwd(), tgt.total_voxels, grid.total_voxels),
+                                  (pack._map_bwd(), grid.total_voxels, tgt.total_voxels)):
+                km = kmap[:n].cpu().numpy()
+                assert int(km.max()) < n_in
+                x = torch.randn(n_in, 64, generator=g).to("cuda", torch.bfloat16)
+                a = (0.1 * torch.randn(kmap.shape[1], 32, 64, generator=g)).to("cuda", torch.bfloat16)
+                ref = fvdb.ext.implicit_gemm(x, a, kmap, n)
+                for win in (0, 256, 2048, 8192):
+                    o = fvdb.ext.row_order(kmap, n, win)
+                    assert np.array_equal(o.cpu().numpy(), oracle.row_order(km, win)), (n, ks, win)
+                    assert torch.equal(fvdb.ext.implicit_gemm(x, a, kmap, n, o), ref)
+    kmap = small.sparse_conv_kernel_map(3, 1)[0]._map_fwd()
+    for bad in (-1, 1, 100, 16384):
+        with pytest.raises(ValueError):
+            fvdb.ext.row_order(kmap, small.total_voxels, bad)
+
+
+def test_automatic_row_order_policy():
+    """Without a sorted build the pack orders rows by tap mask per window where that pays
+    (fp32; 16-bit at 128+ input channels; maps with one tap per row), and the results do
+    not depend on it: bit-identical to tree order, forward and backward."""
+    fvdb = _fvdb()
+    from fvdb.utils.synthetic import sphere_points
+
+    grid = fvdb.GridBatch.from_points(sphere_points(seed=2, n_points=100_000, device="cuda"), voxel_sizes=1 / 128)
+    p3, _ = grid.sparse_conv_kernel_map(3, 1)
+    p2, coarse = grid.sparse_conv_kernel_map(2, 2)
+    p3.build_implicit_gemm(training=True)
+    p2.build_implicit_gemm(training=True)
+    bf, f32 = torch.bfloat16, torch.float32
+    assert p3._order_for(True, 64, bf) is None and p3._order_for(True, 64, torch.float64) is None
+    assert p3._order_for(True, 64, f32) is not None and p3._order_for(True, 128, bf) is not None
+    assert p2._order_for(True, 64, bf) is None          # coarse rows: 1..8 taps each
+    o = p2._order_for(False, 64, bf)                   # fine rows: exactly one tap each
+    assert o is not None
+    assert np.array_equal(o.cpu().numpy(), oracle.row_order(p2._map_bwd()[:grid.total_voxels].cpu().numpy(), 8192))
+    g = torch.Generator().manual_seed(8)
+    for dtype in (bf, f32):
+        for pack, n_in, transposed, cin, cout in ((p3, grid.total_voxels, False, 128, 64),
+                                                  (p2, coarse.total_voxels, True, 64, 32)):
+            shape = (cin, cout, *pack.kernel_size) if transposed else (cout, cin, *pack.kernel_size)
+            w = (0.1 * torch.randn(*shape, generator=g)).to("cuda", dtype)
+            x = torch.randn(n_in, cin, generator=g).to("cuda", dtype)
+            outs = []
+            for auto in (True, False):
+                if not auto:
+                    pack._order_for = lambda fwd, c, dt: None  # instance attribute shadows the method
+                try:
+                    xx, ww = x.clone().requires_grad_(True), w.clone().requires_grad_(True)
+                    f = pack.sparse_transpose_conv_3d if transposed else pack.sparse_conv_3d
+                    y = f(xx, ww, fvdb.ConvPackBackend.IGEMM).jdata
+                    y.backward(torch.linspace(-1, 1, y.numel(), device="cuda").reshape(y.shape).to(dtype))
+                    outs.append((y.detach(), xx.grad, ww.grad))
+                finally:
+                    pack.__dict__.pop("_order_for", None)
+            for a, b in zip(*outs):
+                assert torch.equal(a, b)
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16])
