@@ -201,9 +201,10 @@ def main() -> int:
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only loop the dominant kernel (for profiler runs)")
-    ap.add_argument("--workload", default="precision", choices=["precision", "single", "diff", "unet", "serve"],
+    ap.add_argument("--workload", default="precision", choices=["precision", "single", "diff", "unet", "serve", "orders"],
                     help="precision (default, the headline line); single / diff / unet / serve: the other "
-                         "workloads, one JSON line each")
+                         "workloads, one JSON line each; orders: contraction per layer in tree order vs "
+                         "tap-mask order per window")
     ap.add_argument("--requests", type=int, default=120, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: Poisson arrival rate (req/s); 0 = auto")
     args = ap.parse_args()
@@ -219,7 +220,8 @@ def main() -> int:
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     if args.workload != "precision":
-        rec = {"single": run_single, "diff": run_diff, "unet": run_unet, "serve": run_serve}[args.workload](
+        rec = {"single": run_single, "diff": run_diff, "unet": run_unet, "serve": run_serve,
+               "orders": run_orders}[args.workload](
             torch, fvdb, dev, args)
         print(json.dumps(rec), flush=True)
         return 0
@@ -513,6 +515,62 @@ def run_diff(torch, fvdb, dev, args):
     return {"workload": "diff: planes+spheres at 1/1024, 3x3x3 128->128, forward + dgrad + wgrad", "voxels": N,
             "pairs": P, "flops_per_pass": F, "timings": out, "errors_vs_fp64_cpu_oracle": errs,
             "fd_check": {"subgrid_voxels": n, "channels": 8, "h": h, "max_rel_err": fd_err, "pass": fd_err < 1e-6}}
+
+
+def run_orders(torch, fvdb, dev, args):
+    """Contraction time per layer shape in tree order and in tap-mask order per 8192-row window (what packs
+    pick by themselves where ``wants_row_order`` says so), bf16 and fp32, on the precision, differentiation
+    and U-Net grids; outputs are checked bit-identical between the two orders."""
+    from fvdb.sparse_conv_pack_info import AUTO_ORDER_WINDOW, wants_row_order
+    from fvdb.utils.synthetic import planes_spheres_points
+
+    ext = fvdb.ext
+    rows = []
+
+    def case(name, m, n_rows, n_in, cin, cout, one_tap=False):
+        K = m.shape[1]
+        g = torch.Generator(device=dev).manual_seed(3)
+        x64 = torch.randn((n_in, cin), generator=g, device=dev, dtype=torch.float64)
+        a64 = torch.randn((K, cout, cin), generator=g, device=dev, dtype=torch.float64) / math.sqrt(cin * K)
+        order = ext.row_order(m, n_rows, AUTO_ORDER_WINDOW)
+        rec = {"layer": name, "rows": n_rows, "taps": K, "order_ms": _median_ms(torch, lambda: ext.row_order(
+            m, n_rows, AUTO_ORDER_WINDOW), 10)}
+        for dn, dt in (("bf16", torch.bfloat16), ("fp32", torch.float32)):
+            x, a = x64.to(dt), a64.to(dt)
+            ref = ext.implicit_gemm(x, a, m, n_rows)
+            if not torch.equal(ext.implicit_gemm(x, a, m, n_rows, order), ref):
+                raise RuntimeError(f"{name} {dn}: row order changed the result")
+            rec[dn] = {"tree_ms": _median_ms(torch, lambda: ext.implicit_gemm(x, a, m, n_rows), 10),
+                       "windowed_ms": _median_ms(torch, lambda: ext.implicit_gemm(x, a, m, n_rows, order), 10),
+                       "picked": "windowed" if wants_row_order(K, cin, dt, one_tap) else "tree"}
+        log(f"[bench] orders {rec}")
+        rows.append(rec)
+
+    grid, _, _, _, _ = build_workload(torch, fvdb, dev)
+    pa, _ = grid.sparse_conv_kernel_map(3, 1)
+    case("sphere batch 3x3x3 64->64", pa._map_fwd(), grid.total_voxels, grid.total_voxels, 64, 64)
+    del grid, pa
+    pts = planes_spheres_points(seed=7, n_points=2_400_000, n_planes=2, n_spheres=2, device=dev)
+    gd = fvdb.GridBatch.from_points(pts, voxel_sizes=1 / 1024)
+    pd, _ = gd.sparse_conv_kernel_map(3, 1)
+    case("planes+spheres 3x3x3 128->128", pd._map_fwd(), gd.total_voxels, gd.total_voxels, 128, 128)
+    del gd, pd
+    gl = _indoor_batch(torch, fvdb, dev, [1, 2, 3, 4])
+    chans = (32, 64, 128, 256)
+    for l, c in enumerate(chans):
+        n = gl.total_voxels
+        pk, _ = gl.sparse_conv_kernel_map(3, 1)
+        if l == 0 or l == 3:
+            case(f"U-Net level {l} 3x3x3 {c}->{c}", pk._map_fwd(), n, n, c, c)
+        else:
+            case(f"U-Net level {l} decoder 3x3x3 {2 * c}->{c}", pk._map_fwd(), n, n, 2 * c, c)
+        if l + 1 < len(chans):
+            pdn, gc = gl.sparse_conv_kernel_map(2, 2)
+            case(f"U-Net level {l} up 2x2x2 s2 transposed {chans[l + 1]}->{c}", pdn._map_bwd(), n, gc.total_voxels,
+                 chans[l + 1], c, one_tap=True)
+            gl = gc
+    return {"workload": "orders: contraction per layer, tree order vs tap-mask order per 8192-row window",
+            "layers": rows}
 
 
 def _indoor_batch(torch, fvdb, dev, seeds):
