@@ -475,13 +475,31 @@ void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32
         if (v + 2 < kv) fetch(v + 2, w[(u + 2) % 3]);
         if (v + LX < kv) gather(v + LX, b[(u + LX) % (LX + 1)]);
         const T* wb = sW + (u % 2) * WT + r16 * WP + g * F::EPL;
+        if constexpr (SKIP && (WPE > 0 || MI >= 8)) {
+            // Under the 4-waves-per-SIMD register budget, and for 128-channel tiles, the
+            // loop's scheduler would read one weight fragment at a time, each waiting for
+            // the last: read all fragments of a channel chunk before its MFMAs so the LDS
+            // reads overlap (measured on MI355X; the other shapes schedule better alone).
 #pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) {
+            for (int ch = 0; ch < NCH; ++ch) {
+                V a[MI];
 #pragma unroll
-            for (int i = 0; i < MI; ++i) {
-                const V a = *reinterpret_cast<const V*>(wb + i * 16 * WP + ch * F::CH);
+                for (int i = 0; i < MI; ++i) a[i] = *reinterpret_cast<const V*>(wb + i * 16 * WP + ch * F::CH);
+                __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the MFMAs
 #pragma unroll
-                for (int j = 0; j < NI; ++j) mfma(a, b[u % (LX + 1)][j][ch], acc[i][j]);
+                for (int i = 0; i < MI; ++i)
+#pragma unroll
+                    for (int j = 0; j < NI; ++j) mfma(a[i], b[u % (LX + 1)][j][ch], acc[i][j]);
+            }
+        } else {
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) {
+#pragma unroll
+                for (int i = 0; i < MI; ++i) {
+                    const V a = *reinterpret_cast<const V*>(wb + i * 16 * WP + ch * F::CH);
+#pragma unroll
+                    for (int j = 0; j < NI; ++j) mfma(a, b[u % (LX + 1)][j][ch], acc[i][j]);
+                }
             }
         }
         if (v + 1 < kv) {
