@@ -272,7 +272,9 @@ def main() -> int:
         return lambda: pb.sparse_conv_3d(pa.sparse_conv_3d(xj, w1, IG), w2, IG)
 
     log("[bench] timing conv-only")
-    tc = {name: time_steps(torch, conv_only(dt), args.steps, args.warmup)
+    # median of per-iteration HIP-event times on the launch stream (host wall clock
+    # around a few launches is noisy at this scale)
+    tc = {name: event_avg_ms(torch, conv_only(dt), max(10, args.steps))[0] * 1e-3
           for name, dt in (("bf16", torch.bfloat16), ("fp16", torch.float16), ("fp32", torch.float32))}
 
     log("[bench] timing dominant kernel with HIP events")

@@ -91,7 +91,8 @@ __global__ __launch_bounds__(256) void k_conv_tile(const T* __restrict__ X, cons
         for (int q = 0; q < SEG_PER_T; ++q) {
             const int sgi = tid + q * 256;
             const int m = sgi / (CIN / F::EPL), c = (sgi - m * (CIN / F::EPL)) * F::EPL;
-            if (sgi < SEGS && m0 + m < M) w[q] = *reinterpret_cast<const V*>(A + ((int64_t)k * M + m0 + m) * CIN + c);
+            const int mc = m0 + tile_row_channel<MI>(m);
+            if (sgi < SEGS && mc < M) w[q] = *reinterpret_cast<const V*>(A + ((int64_t)k * M + mc) * CIN + c);
             else w[q] = zero_frag<T>();
         }
     };
@@ -183,21 +184,15 @@ __global__ __launch_bounds__(256) void k_conv_tile(const T* __restrict__ X, cons
         }
     }
 
-    // lane (g, r16) of tile (i, j) holds D[m = 4g + e][row = r16]
+    // lane (g, r16) of tile (i, j) holds D[4g + e][row = r16], output channel
+    // m0 + g*4*MI + 4i + e (the weight tile's rows were staged in that order)
+    const bool vec_ok = store_vec_ok<T, MI>(Y, M);
 #pragma unroll
     for (int j = 0; j < NI; ++j) {
         const int64_t ri = row0 + wave * NI * 16 + j * 16 + r16;
         if (ri >= n_rows) continue;
         const int64_t r = perm ? (int64_t)perm[ri] : ri;
-        T* __restrict__ y = Y + r * M;
-#pragma unroll
-        for (int i = 0; i < MI; ++i) {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int m = m0 + i * 16 + 4 * g + e;
-                if (m < M) y[m] = epilogue<T>(acc[i][j][e], bias, m, relu);
-            }
-        }
+        store_channels<T, MI>(Y + r * M, acc, j, m0 + g * 4 * MI, M, bias, relu, vec_ok);
     }
 }
 
@@ -258,8 +253,9 @@ void k_conv_tile_k(const T* __restrict__ X, const T* __restrict__ A,
         for (int q = 0; q < SEG_PER_T; ++q) {
             const int sgi = tid + q * 256;
             const int m = sgi / (CIN / F::EPL), c = (sgi - m * (CIN / F::EPL)) * F::EPL;
-            const bool ok = sgi < SEGS && m0 + m < M;
-            const V v = *reinterpret_cast<const V*>(A + ((int64_t)k * M + (ok ? m0 + m : 0)) * CIN + (ok ? c : 0));
+            const int mc = m0 + tile_row_channel<MI>(m);
+            const bool ok = sgi < SEGS && mc < M;
+            const V v = *reinterpret_cast<const V*>(A + ((int64_t)k * M + (ok ? mc : 0)) * CIN + (ok ? c : 0));
             wr[q] = ok ? v : zero_frag<T>();
         }
     };
@@ -319,19 +315,12 @@ void k_conv_tile_k(const T* __restrict__ X, const T* __restrict__ A,
         }
     }
 
+    const bool vec_ok = store_vec_ok<T, MI>(Y, M);
 #pragma unroll
     for (int j = 0; j < NI; ++j) {
         const int64_t ri = row0 + wave * NI * 16 + j * 16 + r16;
         if (ri >= n_rows) continue;
-        T* __restrict__ y = Y + ri * M;
-#pragma unroll
-        for (int i = 0; i < MI; ++i) {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int m = m0 + i * 16 + 4 * g + e;
-                if (m < M) y[m] = epilogue<T>(acc[i][j][e], bias, m, relu);
-            }
-        }
+        store_channels<T, MI>(Y + ri * M, acc, j, m0 + g * 4 * MI, M, bias, relu, vec_ok);
     }
 }
 
@@ -438,9 +427,10 @@ void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32
         for (int q = 0; q < SEG_PER_T; ++q) {
             const int sgi = tid + q * 256;
             const int m = sgi / (CC / F::EPL), c = (sgi - m * (CC / F::EPL)) * F::EPL;
-            const bool ok = sgi < SEGS && m0 + m < M;
+            const int mc = m0 + tile_row_channel<MI>(m);
+            const bool ok = sgi < SEGS && mc < M;
             const V vv =
-                *reinterpret_cast<const V*>(A + ((int64_t)tap * M + (ok ? m0 + m : 0)) * CIN + c0 + (ok ? c : 0));
+                *reinterpret_cast<const V*>(A + ((int64_t)tap * M + (ok ? mc : 0)) * CIN + c0 + (ok ? c : 0));
             wr[q] = ok ? vv : zero_frag<T>();
         }
     };
@@ -536,20 +526,13 @@ void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32
         for (int v0 = 0; v0 < kv; v0 += U) six(v0);
     }
 
+    const bool vec_ok = store_vec_ok<T, MI>(Y, M);
 #pragma unroll
     for (int j = 0; j < NI; ++j) {
         const int64_t ri = row0 + wave * NI * 16 + j * 16 + r16;
         if (ri >= n_rows) continue;
         const int64_t r = perm ? (int64_t)perm[ri] : ri;
-        T* __restrict__ y = Y + r * M;
-#pragma unroll
-        for (int i = 0; i < MI; ++i) {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int m = m0 + i * 16 + 4 * g + e;
-                if (m < M) y[m] = epilogue<T>(acc[i][j][e], bias, m, relu);
-            }
-        }
+        store_channels<T, MI>(Y + r * M, acc, j, m0 + g * 4 * MI, M, bias, relu, vec_ok);
     }
 }
 
