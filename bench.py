@@ -244,11 +244,12 @@ def main() -> int:
     fmap = pa._map_fwd()
     conv_a = lambda: fvdb.ext.implicit_gemm(xb, kio, fmap, N)  # noqa: E731
     if args.kernel_only:
-        # a plain device copy of known size first: its FETCH_SIZE / WRITE_SIZE readings
-        # calibrate the HBM-traffic counters of the conv kernel
+        # a streaming elementwise kernel of known size first (reads and writes COPY_BYTES
+        # once; a same-dtype copy_ may run on a DMA engine, which the counters do not
+        # see): its FETCH_SIZE / WRITE_SIZE readings calibrate the conv kernel's
         src = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.neg(src, out=dst)
         torch.cuda.synchronize()
         for i in range(args.steps):
             conv_a()

@@ -56,57 +56,6 @@ __device__ inline T epilogue(float v, const T* __restrict__ bias, int m, int rel
     return from_f32<T>(v);
 }
 
-// Output-channel layout of an MI x 16-channel tile.  Lane group g (lanes 16g..16g+15)
-// of a 16x16 MFMA holds accumulator rows 4g..4g+3 of each block i.  Staging the
-// weight tile so that LDS row i*16 + 4g + e carries channel g*4*MI + 4i + e gives
-// every lane 4*MI consecutive output channels of its row, written with vector stores.
-template <int MI>
-__device__ inline int tile_row_channel(int m) {
-    return ((m & 15) >> 2) * 4 * MI + (m >> 4) * 4 + (m & 3);
-}
-
-template <typename T, int E>
-struct StoreVec {
-    typedef T type __attribute__((ext_vector_type(E)));
-};
-
-// Rounds and writes the 4*MI channels [c0, c0 + 4*MI) of one output row from the
-// accumulators acc[i][e] (channel c0 + 4i + e), with the bias/ReLU epilogue.  Full
-// groups with a 16- or 8-byte aligned destination are written as vectors.
-template <typename T, int MI, typename Acc>
-__device__ inline void store_channels(T* __restrict__ y, const Acc& acc, int j, int c0, int M,
-                                      const T* __restrict__ bias, int relu, bool vec_ok) {
-    constexpr int NC = 4 * MI;
-    constexpr int VB = (NC * (int)sizeof(T)) % 16 == 0 ? 16 : ((NC * (int)sizeof(T)) % 8 == 0 ? 8 : 0);
-    if (VB > 0 && vec_ok && c0 + NC <= M) {
-        constexpr int E = VB > 0 ? VB / (int)sizeof(T) : 1;
-        using VT = typename StoreVec<T, E>::type;
-#pragma unroll
-        for (int v = 0; v < NC / E; ++v) {
-            VT o;
-#pragma unroll
-            for (int q = 0; q < E; ++q) {
-                const int idx = v * E + q;
-                o[q] = epilogue<T>(acc[idx >> 2][j][idx & 3], bias, c0 + idx, relu);
-            }
-            *reinterpret_cast<VT*>(y + c0 + v * E) = o;
-        }
-        return;
-    }
-#pragma unroll
-    for (int idx = 0; idx < NC; ++idx)
-        if (c0 + idx < M) y[c0 + idx] = epilogue<T>(acc[idx >> 2][j][idx & 3], bias, c0 + idx, relu);
-}
-
-// Whether rows of M elements starting at Y can take vector stores of the tile's width.
-template <typename T, int MI>
-__device__ inline bool store_vec_ok(const T* Y, int M) {
-    constexpr int NC = 4 * MI;
-    constexpr int VB = (NC * (int)sizeof(T)) % 16 == 0 ? 16 : ((NC * (int)sizeof(T)) % 8 == 0 ? 8 : 0);
-    if (VB == 0) return false;
-    return ((reinterpret_cast<uintptr_t>(Y) | (uintptr_t)((size_t)M * sizeof(T))) & (uintptr_t)(VB - 1)) == 0;
-}
-
 template <typename T, bool VEC>
 __device__ inline typename Frag<T>::vec load_frag(const T* __restrict__ p, int c, int cin) {
     using V = typename Frag<T>::vec;

@@ -25,6 +25,72 @@ namespace fvdbk {
 
 constexpr int TILE_MAX_K = 32;  // taps whose map tile is staged in LDS (3^3, 2^3, ...)
 
+template <typename T, int E>
+struct StoreVec {
+    typedef T type __attribute__((ext_vector_type(E)));
+};
+
+// Output tile of one wavefront (NI*16 rows x MI*16 channels) in LDS, pitch padded by
+// 16 bytes.  The kernels reuse their weight/map LDS for it after the main loop.
+template <typename T, int MI, int NI>
+constexpr size_t out_tile_lds_bytes() {
+    return (size_t)4 * NI * 16 * (MI * 16 + 16 / sizeof(T)) * sizeof(T);
+}
+
+// Epilogue of the tiled kernels: the wavefront's accumulators (lane (g, r16) of block
+// (i, j) holds channel m0 + i*16 + 4g + e of row j*16 + r16) get the bias/ReLU, are
+// rounded and written to its LDS slice, then read back row-major so that each store
+// instruction writes whole rows (16 bytes per lane, consecutive lanes along a row).
+// Rows at or beyond n_rows are skipped; perm maps tile rows to output rows.  The
+// caller has made the LDS free (barrier after the last read of the weight tiles).
+template <typename T, int MI, int NI>
+__device__ inline void store_tile(T* __restrict__ Y, const floatx4 (&acc)[MI][NI], unsigned char* smem, int wave,
+                                  int lane, int64_t row_base, const int32_t* __restrict__ perm, int64_t n_rows, int M,
+                                  int m0, const T* __restrict__ bias, int relu) {
+    constexpr int TW = MI * 16;
+    constexpr int OP = TW + 16 / (int)sizeof(T);
+    constexpr int RW = NI * 16;
+    constexpr int E = 16 / (int)sizeof(T);  // elements per 16-byte lane segment
+    constexpr int LPR = TW / E;             // lanes per row
+    constexpr int RPP = 64 / LPR;           // rows per pass
+    static_assert(LPR <= 64 && RW % RPP == 0, "output tile shape");
+    using V4 = typename StoreVec<T, 4>::type;
+    using VE = typename StoreVec<T, E>::type;
+    T* s = reinterpret_cast<T*>(smem) + (size_t)wave * RW * OP;
+    const int g = lane >> 4, r16 = lane & 15;
+#pragma unroll
+    for (int j = 0; j < NI; ++j)
+#pragma unroll
+        for (int i = 0; i < MI; ++i) {
+            V4 v;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int m = m0 + i * 16 + 4 * g + e;
+                v[e] = m < M ? epilogue<T>(acc[i][j][e], bias, m, relu) : (T)0.f;
+            }
+            *reinterpret_cast<V4*>(s + (j * 16 + r16) * OP + i * 16 + 4 * g) = v;
+        }
+    const bool vec = ((reinterpret_cast<uintptr_t>(Y) | (uintptr_t)((size_t)M * sizeof(T))) & 15u) == 0;
+    const int rr = lane / LPR, cc = (lane % LPR) * E;
+    const int c = m0 + cc;
+#pragma unroll
+    for (int p = 0; p < RW / RPP; ++p) {
+        const int lr = p * RPP + rr;
+        const int64_t ri = row_base + lr;
+        if (ri >= n_rows) continue;
+        const int64_t r = perm ? (int64_t)perm[ri] : ri;
+        const VE v = *reinterpret_cast<const VE*>(s + lr * OP + cc);
+        T* __restrict__ y = Y + r * M + c;
+        if (vec && c + E <= M) {
+            *reinterpret_cast<VE*>(y) = v;
+        } else {
+#pragma unroll
+            for (int q = 0; q < E; ++q)
+                if (c + q < M) y[q] = v[q];
+        }
+    }
+}
+
 template <typename T, int MI, int NI, int CIN, int LA>
 __global__ __launch_bounds__(256) void k_conv_tile(const T* __restrict__ X, const T* __restrict__ A,
                                                    const int32_t* __restrict__ map,
@@ -91,7 +157,7 @@ __global__ __launch_bounds__(256) void k_conv_tile(const T* __restrict__ X, cons
         for (int q = 0; q < SEG_PER_T; ++q) {
             const int sgi = tid + q * 256;
             const int m = sgi / (CIN / F::EPL), c = (sgi - m * (CIN / F::EPL)) * F::EPL;
-            const int mc = m0 + tile_row_channel<MI>(m);
+            const int mc = m0 + m;
             if (sgi < SEGS && mc < M) w[q] = *reinterpret_cast<const V*>(A + ((int64_t)k * M + mc) * CIN + c);
             else w[q] = zero_frag<T>();
         }
@@ -184,16 +250,8 @@ __global__ __launch_bounds__(256) void k_conv_tile(const T* __restrict__ X, cons
         }
     }
 
-    // lane (g, r16) of tile (i, j) holds D[4g + e][row = r16], output channel
-    // m0 + g*4*MI + 4i + e (the weight tile's rows were staged in that order)
-    const bool vec_ok = store_vec_ok<T, MI>(Y, M);
-#pragma unroll
-    for (int j = 0; j < NI; ++j) {
-        const int64_t ri = row0 + wave * NI * 16 + j * 16 + r16;
-        if (ri >= n_rows) continue;
-        const int64_t r = perm ? (int64_t)perm[ri] : ri;
-        store_channels<T, MI>(Y + r * M, acc, j, m0 + g * 4 * MI, M, bias, relu, vec_ok);
-    }
+    __syncthreads();  // every wavefront is done with the weight tiles
+    store_tile<T, MI, NI>(Y, acc, tile_smem, wave, lane, row0 + wave * NI * 16, perm, n_rows, M, m0, bias, relu);
 }
 
 // Fully unrolled variant for a compile-time kernel volume KT (27 = 3^3, 8 = 2^3): the
@@ -253,7 +311,7 @@ void k_conv_tile_k(const T* __restrict__ X, const T* __restrict__ A,
         for (int q = 0; q < SEG_PER_T; ++q) {
             const int sgi = tid + q * 256;
             const int m = sgi / (CIN / F::EPL), c = (sgi - m * (CIN / F::EPL)) * F::EPL;
-            const int mc = m0 + tile_row_channel<MI>(m);
+            const int mc = m0 + m;
             const bool ok = sgi < SEGS && mc < M;
             const V v = *reinterpret_cast<const V*>(A + ((int64_t)k * M + (ok ? mc : 0)) * CIN + (ok ? c : 0));
             wr[q] = ok ? v : zero_frag<T>();
@@ -315,13 +373,8 @@ void k_conv_tile_k(const T* __restrict__ X, const T* __restrict__ A,
         }
     }
 
-    const bool vec_ok = store_vec_ok<T, MI>(Y, M);
-#pragma unroll
-    for (int j = 0; j < NI; ++j) {
-        const int64_t ri = row0 + wave * NI * 16 + j * 16 + r16;
-        if (ri >= n_rows) continue;
-        store_channels<T, MI>(Y + ri * M, acc, j, m0 + g * 4 * MI, M, bias, relu, vec_ok);
-    }
+    __syncthreads();  // every wavefront is done with the weight tiles
+    store_tile<T, MI, NI>(Y, acc, tile_smem, wave, lane, row0 + wave * NI * 16, nullptr, n_rows, M, m0, bias, relu);
 }
 
 // Wide-channel form (CIN = NCC x CC): each pipeline step is one (tap, channel chunk)
@@ -427,7 +480,7 @@ void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32
         for (int q = 0; q < SEG_PER_T; ++q) {
             const int sgi = tid + q * 256;
             const int m = sgi / (CC / F::EPL), c = (sgi - m * (CC / F::EPL)) * F::EPL;
-            const int mc = m0 + tile_row_channel<MI>(m);
+            const int mc = m0 + m;
             const bool ok = sgi < SEGS && mc < M;
             const V vv =
                 *reinterpret_cast<const V*>(A + ((int64_t)tap * M + (ok ? mc : 0)) * CIN + c0 + (ok ? c : 0));
@@ -526,20 +579,15 @@ void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32
         for (int v0 = 0; v0 < kv; v0 += U) six(v0);
     }
 
-    const bool vec_ok = store_vec_ok<T, MI>(Y, M);
-#pragma unroll
-    for (int j = 0; j < NI; ++j) {
-        const int64_t ri = row0 + wave * NI * 16 + j * 16 + r16;
-        if (ri >= n_rows) continue;
-        const int64_t r = perm ? (int64_t)perm[ri] : ri;
-        store_channels<T, MI>(Y + r * M, acc, j, m0 + g * 4 * MI, M, bias, relu, vec_ok);
-    }
+    __syncthreads();  // every wavefront is done with the weight tiles
+    store_tile<T, MI, NI>(Y, acc, tile_smem, wave, lane, row0 + wave * NI * 16, perm, n_rows, M, m0, bias, relu);
 }
 
 template <typename T, int MI, int NI, int CIN>
 static size_t tile_lds_bytes(int K) {
     constexpr int WP = CIN + 16 / (int)sizeof(T);
-    return (size_t)2 * MI * 16 * WP * sizeof(T) + (size_t)4 * NI * 16 * K * 4;
+    const size_t main_loop = (size_t)2 * MI * 16 * WP * sizeof(T) + (size_t)4 * NI * 16 * K * 4;
+    return main_loop > out_tile_lds_bytes<T, MI, NI>() ? main_loop : out_tile_lds_bytes<T, MI, NI>();
 }
 
 template <typename T, int MI, int NI, int CIN>
