@@ -530,7 +530,7 @@ def run_orders(torch, fvdb, dev, args):
     ext = fvdb.ext
     rows = []
 
-    def case(name, m, n_rows, n_in, cin, cout, one_tap=False):
+    def case(name, m, n_rows, n_in, cin, cout):
         K = m.shape[1]
         g = torch.Generator(device=dev).manual_seed(3)
         x64 = torch.randn((n_in, cin), generator=g, device=dev, dtype=torch.float64)
@@ -545,7 +545,7 @@ def run_orders(torch, fvdb, dev, args):
                 raise RuntimeError(f"{name} {dn}: row order changed the result")
             rec[dn] = {"tree_ms": _median_ms(torch, lambda: ext.implicit_gemm(x, a, m, n_rows), 10),
                        "windowed_ms": _median_ms(torch, lambda: ext.implicit_gemm(x, a, m, n_rows, order), 10),
-                       "picked": "windowed" if wants_row_order(K, cin, dt, one_tap) else "tree"}
+                       "picked": "windowed" if wants_row_order(K, cin, dt) else "tree"}
         log(f"[bench] orders {rec}")
         rows.append(rec)
 
@@ -570,7 +570,7 @@ def run_orders(torch, fvdb, dev, args):
         if l + 1 < len(chans):
             pdn, gc = gl.sparse_conv_kernel_map(2, 2)
             case(f"U-Net level {l} up 2x2x2 s2 transposed {chans[l + 1]}->{c}", pdn._map_bwd(), n, gc.total_voxels,
-                 chans[l + 1], c, one_tap=True)
+                 chans[l + 1], c)
             gl = gc
     return {"workload": "orders: contraction per layer, tree order vs tap-mask order per 8192-row window",
             "layers": rows}

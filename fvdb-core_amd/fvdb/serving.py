@@ -339,11 +339,11 @@ class UNetExecutor:
 
             orders = {}
 
-            def layer(inp, wb, m, n_rows, one_tap=False):
+            def layer(inp, wb, m, n_rows):
                 a, b = wb
                 # rows grouped by tap mask per window where that pays (same rule as the packs)
                 order = None
-                if n_rows > 0 and wants_row_order(m.shape[1], inp.shape[1], inp.dtype, one_tap):
+                if n_rows > 0 and wants_row_order(m.shape[1], inp.shape[1], inp.dtype):
                     if id(m) not in orders:
                         orders[id(m)] = ext.row_order(m, n_rows, AUTO_ORDER_WINDOW)
                     order = orders[id(m)]
@@ -359,7 +359,7 @@ class UNetExecutor:
                 skips.append(x)
             for i, l in enumerate(range(L - 2, -1, -1)):
                 n_l = offs[l][-1]
-                x = layer(x, self.up[i], up[l], n_l, one_tap=True)
+                x = layer(x, self.up[i], up[l], n_l)
                 x = torch.cat([x, skips[l]], 1)
                 for wb in self.dec[i]:
                     x = layer(x, wb, conv[l], n_l)

@@ -53,14 +53,16 @@ def _pad128(n: int) -> int:
 AUTO_ORDER_WINDOW = 8192
 
 
-def wants_row_order(kernel_volume: int, cin: int, dtype: torch.dtype, one_tap_rows: bool) -> bool:
+def wants_row_order(kernel_volume: int, cin: int, dtype: torch.dtype) -> bool:
     """Whether a contraction runs faster with its rows ordered by tap mask per window
-    (measured on MI355X): fp32 (MFMA-bound, so each skipped tap is saved time), and
-    16-bit inputs of 128+ channels or maps with one tap per row.  Only the kernel
-    volumes with tap-skipping kernels (27, 8) qualify; fp64 runs the scalar kernel."""
+    (measured on MI355X): fp32 (MFMA-bound, so each skipped tap is saved time) and
+    16-bit inputs of 128+ channels.  Narrower 16-bit layers stay in tree order, whose
+    gathers hit cache; that includes maps with one tap per row (a 2x2x2 stride-2
+    kernel's fine-to-coarse map), where tree order computes 8 taps per row.  Only the
+    kernel volumes with tap-skipping kernels (27, 8) qualify; fp64 runs the scalar kernel."""
     if kernel_volume not in (8, 27) or dtype == torch.float64:
         return False
-    return dtype == torch.float32 or cin >= 128 or one_tap_rows
+    return dtype == torch.float32 or cin >= 128
 
 
 class SparseConvPackInfo:
@@ -153,18 +155,14 @@ class SparseConvPackInfo:
     # Rows of a tile that share taps let the contraction skip the taps none of them uses.
     # Without an explicit sorted build, the rows are ordered by tap mask inside windows
     # of AUTO_ORDER_WINDOW consecutive rows (neighbours stay close in memory) where
-    # ``wants_row_order`` says so.  Maps with one tap per row are the fine-to-coarse maps
-    # of kernels that tile space (e.g. 2x2x2 stride 2), which transposed convolutions and
-    # dgrad use.  Narrow 16-bit layers stay in tree order, whose gathers hit cache.
-    def _one_tap_rows(self, fwd: bool) -> bool:
-        return (not fwd) and self._st == self._ks and self.kernel_volume > 1
-
+    # ``wants_row_order`` says so.  Narrow 16-bit layers stay in tree order, whose
+    # gathers hit cache.
     def _order_for(self, fwd: bool, cin: int, dtype: torch.dtype) -> Optional[torch.Tensor]:
         """Row order for a contraction over the forward (rows = target voxels) or
         transposed (rows = source voxels) map with ``cin`` input channels, or None."""
         if self._sorted:
             return self._order_fwd() if fwd else self._order_bwd()
-        if not wants_row_order(self.kernel_volume, cin, dtype, self._one_tap_rows(fwd)):
+        if not wants_row_order(self.kernel_volume, cin, dtype):
             return None
         n = self._tgt.total_voxels if fwd else self._src.total_voxels
         if n == 0:

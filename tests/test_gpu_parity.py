@@ -435,8 +435,8 @@ def test_row_orders_match_oracle():
 
 def test_automatic_row_order_policy():
     """Without a sorted build the pack orders rows by tap mask per window where that pays
-    (fp32; 16-bit at 128+ input channels; maps with one tap per row), and the results do
-    not depend on it: bit-identical to tree order, forward and backward."""
+    (fp32; 16-bit at 128+ input channels), and the results do not depend on it:
+    bit-identical to tree order, forward and backward."""
     fvdb = _fvdb()
     from fvdb.utils.synthetic import sphere_points
 
@@ -449,7 +449,8 @@ def test_automatic_row_order_policy():
     assert p3._order_for(True, 64, bf) is None and p3._order_for(True, 64, torch.float64) is None
     assert p3._order_for(True, 64, f32) is not None and p3._order_for(True, 128, bf) is not None
     assert p2._order_for(True, 64, bf) is None          # coarse rows: 1..8 taps each
-    o = p2._order_for(False, 64, bf)                   # fine rows: exactly one tap each
+    assert p2._order_for(False, 64, bf) is None         # fine rows (one tap each), narrow 16-bit
+    o = p2._order_for(False, 64, f32)                  # fine rows in fp32: windowed order
     assert o is not None
     assert np.array_equal(o.cpu().numpy(), oracle.row_order(p2._map_bwd()[:grid.total_voxels].cpu().numpy(), 8192))
     g = torch.Generator().manual_seed(8)
