@@ -128,9 +128,9 @@ static void launch_conv_mfma(const T* X, int cin, const T* A, const int32_t* map
 }
 
 template <typename T>
-static int conv_mfma_dispatch(const T* X, int cin, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows,
-                              int K, int M, T* Y, const T* bias, int relu, hipStream_t s) {
-    if (conv_tile_dispatch(X, cin, A, map, perm, n_rows, K, M, Y, bias, relu, s)) {
+static int conv_mfma_dispatch(const T* X, int64_t n_in, int cin, const T* A, const int32_t* map, const int32_t* perm,
+                              int64_t n_rows, int K, int M, T* Y, const T* bias, int relu, hipStream_t s) {
+    if (conv_tile_dispatch(X, n_in, cin, A, map, perm, n_rows, K, M, Y, bias, relu, s)) {
         FVDBK_LAUNCH_CHECK("k_conv_tile");
         return FVDBK_OK;
     }
@@ -408,13 +408,13 @@ static int conv_entry(fvdbk_dtype dtype, const void* in, int64_t n_in, int32_t c
     // paths without tiles ignore it
     switch (dtype) {
     case FVDBK_BF16:
-        return conv_mfma_dispatch<__bf16>((const __bf16*)in, c_in, (const __bf16*)a, map, row_order, n_rows, K, M,
+        return conv_mfma_dispatch<__bf16>((const __bf16*)in, n_in, c_in, (const __bf16*)a, map, row_order, n_rows, K, M,
                                           (__bf16*)out, (const __bf16*)bias, relu, stream);
     case FVDBK_F16:
-        return conv_mfma_dispatch<_Float16>((const _Float16*)in, c_in, (const _Float16*)a, map, row_order, n_rows, K,
+        return conv_mfma_dispatch<_Float16>((const _Float16*)in, n_in, c_in, (const _Float16*)a, map, row_order, n_rows, K,
                                             M, (_Float16*)out, (const _Float16*)bias, relu, stream);
     case FVDBK_F32:
-        return conv_mfma_dispatch<float>((const float*)in, c_in, (const float*)a, map, row_order, n_rows, K, M,
+        return conv_mfma_dispatch<float>((const float*)in, n_in, c_in, (const float*)a, map, row_order, n_rows, K, M,
                                          (float*)out, (const float*)bias, relu, stream);
     case FVDBK_F64: {
         const int64_t n = n_rows * M;

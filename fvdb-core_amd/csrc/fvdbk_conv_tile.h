@@ -43,10 +43,11 @@ constexpr size_t out_tile_lds_bytes() {
 // instruction writes whole rows (16 bytes per lane, consecutive lanes along a row).
 // Rows at or beyond n_rows are skipped; perm maps tile rows to output rows.  The
 // caller has made the LDS free (barrier after the last read of the weight tiles).
+// The wavefront's slice s holds NI*16 rows of MI*16 + 16/sizeof(T) elements.
 template <typename T, int MI, int NI>
-__device__ inline void store_tile(T* __restrict__ Y, const floatx4 (&acc)[MI][NI], unsigned char* smem, int wave,
-                                  int lane, int64_t row_base, const int32_t* __restrict__ perm, int64_t n_rows, int M,
-                                  int m0, const T* __restrict__ bias, int relu) {
+__device__ inline void store_tile_slice(T* __restrict__ Y, const floatx4 (&acc)[MI][NI], T* s, int lane,
+                                        int64_t row_base, const int32_t* __restrict__ perm, int64_t n_rows, int M,
+                                        int m0, const T* __restrict__ bias, int relu) {
     constexpr int TW = MI * 16;
     constexpr int OP = TW + 16 / (int)sizeof(T);
     constexpr int RW = NI * 16;
@@ -56,8 +57,21 @@ __device__ inline void store_tile(T* __restrict__ Y, const floatx4 (&acc)[MI][NI
     static_assert(LPR <= 64 && RW % RPP == 0, "output tile shape");
     using V4 = typename StoreVec<T, 4>::type;
     using VE = typename StoreVec<T, E>::type;
-    T* s = reinterpret_cast<T*>(smem) + (size_t)wave * RW * OP;
     const int g = lane >> 4, r16 = lane & 15;
+    // this lane's bias values (channels m0 + i*16 + 4g + e), loaded together up front
+    float bv[MI][4];
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+        const int mb = m0 + i * 16 + 4 * g;
+        if (bias != nullptr && mb + 4 <= M && (reinterpret_cast<uintptr_t>(bias + mb) & (4 * sizeof(T) - 1)) == 0) {
+            const V4 b4 = *reinterpret_cast<const V4*>(bias + mb);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) bv[i][e] = (float)b4[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) bv[i][e] = (bias != nullptr && mb + e < M) ? (float)bias[mb + e] : 0.f;
+        }
+    }
 #pragma unroll
     for (int j = 0; j < NI; ++j)
 #pragma unroll
@@ -66,7 +80,11 @@ __device__ inline void store_tile(T* __restrict__ Y, const floatx4 (&acc)[MI][NI
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int m = m0 + i * 16 + 4 * g + e;
-                v[e] = m < M ? epilogue<T>(acc[i][j][e], bias, m, relu) : (T)0.f;
+                // the epilogue of epilogue<T>(): bias, ReLU (NaN passes), one rounding
+                float x = acc[i][j][e];
+                if (bias != nullptr) x += bv[i][e];
+                if (relu && !(x > 0.f) && x == x) x = 0.f;
+                v[e] = m < M ? from_f32<T>(x) : (T)0.f;
             }
             *reinterpret_cast<V4*>(s + (j * 16 + r16) * OP + i * 16 + 4 * g) = v;
         }
@@ -89,6 +107,15 @@ __device__ inline void store_tile(T* __restrict__ Y, const floatx4 (&acc)[MI][NI
                 if (c + q < M) y[q] = v[q];
         }
     }
+}
+
+template <typename T, int MI, int NI>
+__device__ inline void store_tile(T* __restrict__ Y, const floatx4 (&acc)[MI][NI], unsigned char* smem, int wave,
+                                  int lane, int64_t row_base, const int32_t* __restrict__ perm, int64_t n_rows, int M,
+                                  int m0, const T* __restrict__ bias, int relu) {
+    constexpr size_t slice = (size_t)NI * 16 * (MI * 16 + 16 / sizeof(T));
+    store_tile_slice<T, MI, NI>(Y, acc, reinterpret_cast<T*>(smem) + wave * slice, lane, row_base, perm, n_rows, M, m0,
+                                bias, relu);
 }
 
 template <typename T, int MI, int NI, int CIN, int LA>
@@ -377,6 +404,196 @@ void k_conv_tile_k(const T* __restrict__ X, const T* __restrict__ A,
     store_tile<T, MI, NI>(Y, acc, tile_smem, wave, lane, row0 + wave * NI * 16, nullptr, n_rows, M, m0, bias, relu);
 }
 
+// Workgroup barrier without a memory fence.  The caller has waited for the counters it
+// needs (s_waitcnt); the scheduling barriers keep every instruction on its side.
+__device__ inline void wg_barrier() {
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("s_barrier" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// Copies 16 bytes per lane from global memory to LDS: lane i's bytes land at
+// lds + 16*i (global_load_lds_dwordx4).  A non-template wrapper: called with pointers
+// whose type depends on a kernel's template parameter, the builtin made the host pass
+// drop the kernel's launch stub.
+__device__ inline void lds_dma16(const void* src, void* lds) {
+    __builtin_amdgcn_global_load_lds(src, lds, 16, 0, 0);
+}
+
+// s_waitcnt vmcnt(N) lgkmcnt(0), N a compile-time count below 64 (gfx9 encoding: vmcnt
+// bits [3:0] and [15:14], expcnt [6:4] left at its maximum, lgkmcnt [11:8]).
+template <int N>
+__device__ inline void wait_vm_lds() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x70);
+}
+// The same for a count that is constant once the caller's loop is unrolled.
+__device__ inline void wait_vm_lds(int n) {
+    switch (n) {
+#define FVDBK_WAIT_CASE(x) \
+    case x: wait_vm_lds<x>(); break;
+        FVDBK_WAIT_CASE(1) FVDBK_WAIT_CASE(2) FVDBK_WAIT_CASE(3) FVDBK_WAIT_CASE(4) FVDBK_WAIT_CASE(5)
+        FVDBK_WAIT_CASE(6) FVDBK_WAIT_CASE(7) FVDBK_WAIT_CASE(8) FVDBK_WAIT_CASE(9) FVDBK_WAIT_CASE(10)
+        FVDBK_WAIT_CASE(11) FVDBK_WAIT_CASE(12) FVDBK_WAIT_CASE(13) FVDBK_WAIT_CASE(14) FVDBK_WAIT_CASE(15)
+        FVDBK_WAIT_CASE(16) FVDBK_WAIT_CASE(17) FVDBK_WAIT_CASE(18) FVDBK_WAIT_CASE(19) FVDBK_WAIT_CASE(20)
+#undef FVDBK_WAIT_CASE
+    default: wait_vm_lds<0>(); break;
+    }
+}
+
+// Direct-to-LDS form of k_conv_tile_k for the most common narrow layer: 16-bit features,
+// 64 input channels, 27 taps, rows in tree order, outputs in whole 64-channel tiles.
+// Same tile (128 rows x 64 output channels, 4 wavefronts of 32 x 64), same MFMA order,
+// so the results are bit-identical to k_conv_tile_k; the memory pipeline differs:
+//  * weights: each tap's 64 x 64 tile is copied global -> LDS by the load unit
+//    (global_load_lds_dwordx4, 2 per lane), two taps ahead into a ring of three
+//    buffers, with no registers and no LDS store instructions.  Rows are 128 bytes
+//    unpadded; the 16-byte segment s of row r sits at position s ^ (r & 7), so the
+//    MFMA operand reads of 8 consecutive rows hit distinct banks.  The buffers are
+//    separate LDS objects so that the compiler can tell the staging target from the
+//    buffer being read and does not wait for every outstanding load.
+//  * gathers: raw buffer loads over the feature rows; a missing neighbour takes an
+//    offset beyond the buffer, which loads zeros without a branch, so every tap issues
+//    exactly NI*2 gathers + 2 staging loads.  With that fixed count the end of each tap
+//    waits (vmcnt) only for the loads the next tap needs, and the gathers run LX = 3
+//    taps ahead within 120 registers (4 waves per SIMD).
+//  * epilogue: the output tile goes through LDS (wavefront slices in the weight
+//    buffers and the map tile) and is stored as whole rows.
+// Features must span fewer than 2^31 bytes (32-bit buffer offsets); callers check.
+// Weight ring of k_conv_tile_dma: three separate LDS objects of 8 KB (file scope: a
+// static __shared__ array inside a kernel template does not link).
+static __shared__ __attribute__((aligned(16))) uint4 dma_ring0[512], dma_ring1[512], dma_ring2[512];
+
+template <typename T, int LX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict__ A, const int32_t* __restrict__ map,
+                     int64_t n_rows, int M, int n_mt, int64_t n_tiles, T* __restrict__ Y, const T* __restrict__ bias,
+                     int relu) {
+    static_assert(sizeof(T) == 2, "16-bit features");
+    using F = Frag<T>;
+    using V = typename F::vec;
+    constexpr int MI = 4, NI = 2, CIN = 64, KT = 27;
+    constexpr int RT = 4 * NI * 16;
+    constexpr int WT = MI * 16 * CIN;          // elements of one tap tile (8 KB)
+    constexpr int NCH = CIN / F::CH;
+    constexpr int GPT = NI * NCH;              // gather loads per tap and lane
+    constexpr int OUT_SLICE = NI * 16 * (MI * 16 + 8);
+    static_assert(OUT_SLICE <= WT && OUT_SLICE <= RT * KT * 2, "output slices fit the weight and map buffers");
+    static_assert(WT * sizeof(T) == sizeof(dma_ring0), "ring buffer size");
+    extern __shared__ __attribute__((aligned(16))) unsigned char tile_smem[];
+    T* const sW0 = reinterpret_cast<T*>(dma_ring0);
+    T* const sW1 = reinterpret_cast<T*>(dma_ring1);
+    T* const sW2 = reinterpret_cast<T*>(dma_ring2);
+    int32_t* sMap = reinterpret_cast<int32_t*>(tile_smem);
+    auto wbuf = [&](int k) -> T* {
+        const int r = k % 3;
+        return r == 0 ? sW0 : (r == 1 ? sW1 : sW2);
+    };
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int64_t per = (n_tiles + 7) / 8;
+    const int64_t t = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (t >= n_tiles) return;
+    const int64_t row0 = (t / n_mt) * RT;
+    const int m0 = (int)(t % n_mt) * (MI * 16);
+    {
+        const int64_t base = row0 * KT;
+        const int64_t lim = n_rows * KT;
+        for (int e = tid; e < RT * KT; e += 256) sMap[e] = (base + e < lim) ? map[base + e] : -1;
+    }
+    floatx4 acc[MI][NI];
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NI; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    // staging: wavefront w copies rows (2w + q)*8 .. +7 of the tile, lane -> (row, position)
+    const T* wsrc[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int row = (wave * 2 + q) * 8 + (lane >> 3);
+        const int seg = (lane & 7) ^ (lane >> 3);
+        wsrc[q] = A + (int64_t)(m0 + row) * CIN + seg * 8;
+    }
+    auto stage = [&](int k) {
+        T* dst = wbuf(k) + wave * 2 * 512;
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            lds_dma16(wsrc[q] + (int64_t)k * M * CIN, dst + q * 512);
+    };
+    const int bytes = (int)(n_in * CIN * (int64_t)sizeof(T));
+    const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, bytes, 0x00020000);
+    constexpr int MISSING = 0x7FFFFFC0;        // beyond any buffer: the load returns zeros
+    V b[LX + 1][NI][NCH];
+    auto gather = [&](int k, V (&bb)[NI][NCH]) {
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+            const int32_t src = sMap[(wave * NI * 16 + j * 16 + r16) * KT + k];
+            const int off = src >= 0 ? src * (CIN * (int)sizeof(T)) + g * 16 : MISSING;
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch)
+                bb[j][ch] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(xr, off + ch * 64, 0, 0));
+        }
+    };
+    // Loads issued after the staging of tap k+1, up to the end of tap k: issue order is
+    // stage 0, stage 1, gathers 0..LX-1, then per tap t: stage t+2, gathers t+LX.
+    auto after_stage = [&](int k) -> int {
+        auto G = [&](int tt) { return tt + LX < KT ? GPT : 0; };
+        auto D = [&](int tt) { return tt + 2 < KT ? 2 : 0; };
+        return k == 0 ? GPT * LX + D(0) + G(0) : G(k - 1) + D(k) + G(k);
+    };
+
+    __syncthreads();  // map tile visible
+    stage(0);
+    stage(1);
+#pragma unroll
+    for (int k = 0; k < LX; ++k) gather(k, b[k]);
+    wait_vm_lds<0>();
+    wg_barrier();
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+        if (k + 2 < KT) stage(k + 2);
+        if (k + LX < KT) gather(k + LX, b[(k + LX) % (LX + 1)]);
+        const T* wb = wbuf(k);
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+#pragma unroll
+            for (int i = 0; i < MI; ++i) {
+                const int pos = (ch * 4 + g) ^ (r16 & 7);
+                const V a = *reinterpret_cast<const V*>(wb + (i * 16 + r16) * CIN + pos * 8);
+#pragma unroll
+                for (int j = 0; j < NI; ++j) mfma(a, b[k % (LX + 1)][j][ch], acc[i][j]);
+            }
+        }
+        if (k + 1 < KT) {
+            // tap k+1's weights have landed in every wavefront; nobody reads tap k's buffer
+            // again before stage(k+3) overwrites it
+            __builtin_amdgcn_sched_barrier(0);
+            wait_vm_lds(after_stage(k));
+            wg_barrier();
+        }
+    }
+    __syncthreads();  // every wavefront is done with the weight buffers and the map tile
+    T* slice = wave == 0 ? sW0 : (wave == 1 ? sW1 : (wave == 2 ? sW2 : reinterpret_cast<T*>(tile_smem)));
+    store_tile_slice<T, MI, NI>(Y, acc, slice, lane, row0 + wave * NI * 16, nullptr, n_rows, M, m0, bias, relu);
+}
+
+// Launches k_conv_tile_dma when the layer fits it; false otherwise.
+template <typename T>
+static bool launch_tile_dma(const T* X, int64_t n_in, const T* A, const int32_t* map, int64_t n_rows, int M, T* Y,
+                            const T* bias, int relu, hipStream_t s) {
+    if (M % 64 != 0 || n_in * 64 * (int64_t)sizeof(T) > 0x7FFFFF00 || ((uintptr_t)Y % 16) != 0) return false;
+    const int64_t n_rt = (n_rows + 127) / 128;
+    const int n_mt = M / 64;
+    const int64_t n_tiles = n_rt * n_mt;
+    const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
+    const size_t lds = (size_t)128 * 27 * 4;  // map tile (the weight ring is static LDS)
+    hipLaunchKernelGGL((k_conv_tile_dma<T, 3>), grid, dim3(256), lds, s, X, n_in, A, map, n_rows, M, n_mt, n_tiles, Y,
+                       bias, relu);
+    return true;
+}
+
 // Wide-channel form (CIN = NCC x CC): each pipeline step is one (tap, channel chunk)
 // pair, chunk fastest, so a step has exactly the registers and LDS of the CC-channel
 // kernel above (weights two steps ahead, gathered rows LX steps ahead, one barrier per
@@ -630,14 +847,16 @@ constexpr size_t TILE_LDS_LIMIT = 65536;
 // Channel tile for a compile-time CIN: 64 output channels x 128 rows when the LDS
 // fits (the best shape measured on MI355X), else 32 x 256 or 16 x 256 rows.
 template <typename T, int CIN>
-static bool try_tile(const T* X, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows, int K, int M,
-                     T* Y, const T* bias, int relu, hipStream_t s) {
+static bool try_tile(const T* X, int64_t n_in, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows,
+                     int K, int M, T* Y, const T* bias, int relu, hipStream_t s) {
     if constexpr (CIN == 64 && sizeof(T) == 2) {
         // 128-channel tiles for outputs in multiples of 128 channels and 27 taps
         if (M % 128 == 0 && K == 27) {
             launch_tile<T, 8, 2, CIN>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
             return true;
         }
+        // tree-order rows, whole 64-channel output tiles: direct-to-LDS weights, deep gathers
+        if (K == 27 && perm == nullptr && launch_tile_dma<T>(X, n_in, A, map, n_rows, M, Y, bias, relu, s)) return true;
     }
     if (M > 32 && tile_lds_bytes<T, 4, 2, CIN>(K) <= TILE_LDS_LIMIT) {
         launch_tile<T, 4, 2, CIN>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
@@ -730,12 +949,13 @@ static bool try_tile_looped(const T* X, const T* A, const int32_t* map, const in
 
 // LDS-tiled path for the channel counts it is compiled for; false if not applicable.
 template <typename T>
-inline bool conv_tile_dispatch_t(const T* X, int cin, const T* A, const int32_t* map, const int32_t* perm,
-                               int64_t n_rows, int K, int M, T* Y, const T* bias, int relu, hipStream_t s) {
+inline bool conv_tile_dispatch_t(const T* X, int64_t n_in, int cin, const T* A, const int32_t* map,
+                                 const int32_t* perm, int64_t n_rows, int K, int M, T* Y, const T* bias, int relu,
+                                 hipStream_t s) {
     if (K > TILE_MAX_K || ((uintptr_t)X % 16) != 0 || ((uintptr_t)A % 16) != 0) return false;
     switch (cin) {
-    case 32: return try_tile<T, 32>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
-    case 64: return try_tile<T, 64>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
+    case 32: return try_tile<T, 32>(X, n_in, A, map, perm, n_rows, K, M, Y, bias, relu, s);
+    case 64: return try_tile<T, 64>(X, n_in, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     case 128:
         if (K == 27 || K == 8) return try_tile_c<T, 2>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
         return try_tile_looped<T, 128>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
@@ -748,11 +968,11 @@ inline bool conv_tile_dispatch_t(const T* X, int cin, const T* A, const int32_t*
 
 // Per element type (one translation unit each): launches the LDS-tiled kernel and
 // returns true, or returns false when the shape has no tiled instance.
-bool conv_tile_dispatch(const __bf16* X, int cin, const __bf16* A, const int32_t* map, const int32_t* perm,
+bool conv_tile_dispatch(const __bf16* X, int64_t n_in, int cin, const __bf16* A, const int32_t* map, const int32_t* perm,
                         int64_t n_rows, int K, int M, __bf16* Y, const __bf16* bias, int relu, hipStream_t s);
-bool conv_tile_dispatch(const _Float16* X, int cin, const _Float16* A, const int32_t* map, const int32_t* perm,
+bool conv_tile_dispatch(const _Float16* X, int64_t n_in, int cin, const _Float16* A, const int32_t* map, const int32_t* perm,
                         int64_t n_rows, int K, int M, _Float16* Y, const _Float16* bias, int relu, hipStream_t s);
-bool conv_tile_dispatch(const float* X, int cin, const float* A, const int32_t* map, const int32_t* perm,
+bool conv_tile_dispatch(const float* X, int64_t n_in, int cin, const float* A, const int32_t* map, const int32_t* perm,
                         int64_t n_rows, int K, int M, float* Y, const float* bias, int relu, hipStream_t s);
 
 }  // namespace fvdbk

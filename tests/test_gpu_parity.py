@@ -400,6 +400,37 @@ def test_sorted_rows_bit_exact(dtype, cin, cout):
                 assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
+@pytest.mark.parametrize("cout", [64, 192])
+def test_direct_lds_tile_bit_exact(dtype, cout):
+    """The direct-to-LDS contraction (16-bit, 64 input channels, 27 taps, rows in tree order,
+    outputs in whole 64-channel tiles) equals the tap-skipping kernel run over an identity row
+    order bit for bit (same accumulation order; a skipped tap adds exact zeros), with and
+    without the fused bias + ReLU, on a ragged last tile, and the fp64 oracle within the
+    16-bit bound."""
+    fvdb = _fvdb()
+    from fvdb._lib import ext
+    from fvdb.utils.synthetic import sphere_points
+
+    grid = fvdb.GridBatch.from_points(sphere_points(seed=5, n_points=60_000, device="cuda"), voxel_sizes=1 / 96)
+    pack, _ = grid.sparse_conv_kernel_map(3, 1)
+    n = grid.total_voxels
+    kmap = pack._map_fwd()
+    print(f"[direct-lds] {dtype} cout={cout} rows={n} (last tile {n % 128} rows)", flush=True)
+    g = torch.Generator().manual_seed(21)
+    x = torch.randn(n, 64, generator=g).to("cuda", dtype)
+    w = (0.05 * torch.randn(cout, 64, 3, 3, 3, generator=g)).to("cuda", dtype)
+    a = fvdb._autograd.weight_to_kio(w, False).transpose(1, 2).contiguous()  # [K, M, C]
+    bias = torch.randn(cout, generator=g).to("cuda", dtype)
+    ident = torch.arange(n, dtype=torch.int32, device="cuda")
+    for b, relu in ((None, False), (bias, True), (bias, False), (None, True)):
+        y = ext.implicit_gemm(x, a, kmap, n, None, b, relu)
+        y_skip = ext.implicit_gemm(x, a, kmap, n, ident, b, relu)
+        assert torch.equal(y, y_skip), (b is not None, relu)
+    ref = oracle.sparse_conv(x.double().cpu(), w.double().cpu(), kmap[:n].cpu().numpy(), n, 1)
+    assert rel_err(ext.implicit_gemm(x, a, kmap, n), ref) < REL_TOL[dtype]
+
+
 def test_row_orders_match_oracle():
     """Global and windowed row orders equal the oracle's (a deterministic sort: descending
     tap mask, ties in row order, windows in place), on one block and on many, and the
