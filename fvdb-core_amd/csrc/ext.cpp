@@ -112,10 +112,14 @@ std::vector<at::Tensor> build_grid(at::Tensor ijk, at::Tensor jidx, int64_t n_gr
     const int nbad = hc.data_ptr<int32_t>()[1];
     TORCH_CHECK_VALUE(nbad == 0, nbad, " voxels have a grid index >= ", n_grids,
                       " or coordinates outside [-2^19, 2^19)");
-    // tree order of the leaves: sort only the unique keys (keys are < 2^61, so the
-    // signed sort is the unsigned order)
-    at::Tensor ukeys = std::get<0>(at::sort(lkeys.slice(0, 0, L), /*stable=*/false, /*dim=*/0, /*descending=*/false))
-                           .contiguous();
+    // tree order of the leaves: radix-sort only the unique keys
+    at::Tensor ukeys = at::empty({L}, o64);
+    const size_t ws_bytes = fvdbk_grid_sort_leaf_keys_workspace(L, (int32_t)n_grids);
+    at::Tensor ws = at::empty({(int64_t)std::max<size_t>(ws_bytes, 1)}, ijk.options().dtype(at::kByte));
+    check_rc(fvdbk_grid_sort_leaf_keys(reinterpret_cast<const uint64_t*>(lkeys.data_ptr<int64_t>()),
+                                       reinterpret_cast<uint64_t*>(ukeys.data_ptr<int64_t>()), L, (int32_t)n_grids,
+                                       ws.data_ptr(), ws_bytes, s),
+             "grid_sort_leaf_keys");
     const int64_t cap = next_pow2(2 * L + 2);
     at::Tensor hash = at::empty({cap, 2}, o64);
     check_rc(fvdbk_grid_hash_build(reinterpret_cast<const uint64_t*>(ukeys.data_ptr<int64_t>()), L,

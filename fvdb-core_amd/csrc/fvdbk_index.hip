@@ -3,12 +3,14 @@
 // and independent of launch shape.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <stdarg.h>
 #include <stdio.h>
 
 #include <string>
 
 #include <hipcub/block/block_radix_sort.hpp>
+#include <hipcub/device/device_radix_sort.hpp>
 
 #include "fvdbk_common.h"
 
@@ -694,6 +696,40 @@ int fvdbk_grid_offsets(const uint64_t* leaf_keys, const int64_t* leaf_incl, int6
     hipLaunchKernelGGL(k_grid_offsets, dim3(grid_1d(n_grids + 1, 64)), dim3(64), 0, stream, leaf_keys, leaf_incl,
                        n_leaves, n_grids, offsets);
     FVDBK_LAUNCH_CHECK("k_grid_offsets");
+    return FVDBK_OK;
+}
+
+// Leaf keys of a batch of n_grids grids use bits 0..50 (root tile, upper and lower
+// node digits) and the grid index from bit 51.
+static int leaf_key_end_bit(int32_t n_grids) {
+    int gb = 0;
+    while ((1 << gb) < n_grids) ++gb;
+    return 51 + gb;
+}
+
+size_t fvdbk_grid_sort_leaf_keys_workspace(int64_t n, int32_t n_grids) {
+    if (n <= 0 || n > INT32_MAX || n_grids < 0 || n_grids > kMaxGrids) return 0;
+    size_t bytes = 0;
+    if (hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, static_cast<const uint64_t*>(nullptr),
+                                          static_cast<uint64_t*>(nullptr), (int)n, 0, leaf_key_end_bit(n_grids),
+                                          (hipStream_t)0) != hipSuccess)
+        return 0;
+    return bytes;
+}
+
+int fvdbk_grid_sort_leaf_keys(const uint64_t* keys_in, uint64_t* keys_out, int64_t n, int32_t n_grids,
+                              void* workspace, size_t workspace_bytes, hipStream_t stream) {
+    FVDBK_REQUIRE(n >= 0 && n <= INT32_MAX, FVDBK_EINVAL, "leaf count %lld out of range", (long long)n);
+    FVDBK_REQUIRE(n_grids >= 0 && n_grids <= kMaxGrids, FVDBK_EINVAL, "a grid batch holds at most %d grids",
+                  kMaxGrids);
+    if (n == 0) return FVDBK_OK;
+    FVDBK_REQUIRE(keys_in && keys_out && keys_in != keys_out, FVDBK_EINVAL, "sort needs distinct key buffers");
+    size_t need = fvdbk_grid_sort_leaf_keys_workspace(n, n_grids);
+    FVDBK_REQUIRE(need > 0 && workspace != nullptr && workspace_bytes >= need, FVDBK_EINVAL,
+                  "sort workspace of %zu bytes is smaller than the %zu needed", workspace_bytes, need);
+    const hipError_t e = hipcub::DeviceRadixSort::SortKeys(workspace, need, keys_in, keys_out, (int)n, 0,
+                                                           leaf_key_end_bit(n_grids), stream);
+    FVDBK_REQUIRE(e == hipSuccess, FVDBK_EHIP, "leaf key sort: %s", hipGetErrorString(e));
     return FVDBK_OK;
 }
 

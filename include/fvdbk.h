@@ -111,6 +111,16 @@ int fvdbk_grid_leaf_dedupe(const int32_t* ijk, const int32_t* batch_idx, int64_t
                            uint64_t* set, int64_t set_cap, uint64_t* leaf_keys, uint32_t* counters,
                            hipStream_t stream);
 
+/* Step 2's sort of the unique leaf keys into tree order: keys_out = keys_in sorted
+ * ascending (keys_out must not overlap keys_in), an LSD radix sort over the key bits a
+ * batch of n_grids grids can use (bits 0..50 plus the grid index bits).  workspace is
+ * device scratch of fvdbk_grid_sort_leaf_keys_workspace(n, n_grids) bytes.  Replaces
+ * the sort inside nanovdb::tools::cuda::voxelsToGrid (src/fvdb/detail/ops/
+ * BuildGridFromIjk.cu:66-81), which orders voxels by the same tree levels. */
+size_t fvdbk_grid_sort_leaf_keys_workspace(int64_t n, int32_t n_grids);
+int fvdbk_grid_sort_leaf_keys(const uint64_t* keys_in, uint64_t* keys_out, int64_t n, int32_t n_grids,
+                              void* workspace, size_t workspace_bytes, hipStream_t stream);
+
 int fvdbk_grid_hash_build(const uint64_t* leaf_keys, int64_t n_leaves, fvdbk_hslot* hash, int64_t hash_cap,
                           hipStream_t stream);
 
