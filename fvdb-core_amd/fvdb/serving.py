@@ -237,13 +237,12 @@ class UNetTopology:
 
 
 def _shift_cat(maps: Sequence[torch.Tensor], offsets: Sequence[int]) -> torch.Tensor:
-    """Concatenate maps, adding offsets[i] to the valid entries of maps[i]."""
+    """Concatenate maps, adding offsets[i] to the valid entries of maps[i] (one elementwise
+    pass per shifted map; no host synchronisation)."""
     if len(maps) == 1:
         return maps[0]
-    out = torch.cat(list(maps), 0)
-    rows = torch.tensor([m.shape[0] for m in maps], device=out.device)
-    shift = torch.repeat_interleave(torch.tensor(list(offsets), device=out.device, dtype=torch.int32), rows)
-    return torch.where(out >= 0, out + shift[:, None], out)
+    parts = [m if int(o) == 0 else torch.where(m >= 0, m + int(o), m) for m, o in zip(maps, offsets)]
+    return torch.cat(parts, 0)
 
 
 class UNetExecutor:
