@@ -439,7 +439,6 @@ def run_diff(torch, fvdb, dev, args):
     out = {}
     grads = {}
     pairs, sizes = pack._pairs()
-    chunk = wgrad_chunk(P)
     for name, dt in (("bf16", torch.bfloat16), ("fp32", torch.float32)):
         x, w, R = x64.to(dt), w64.to(dt), R64.to(dt)
         kio = weight_to_kio(w, False)
@@ -448,6 +447,7 @@ def run_diff(torch, fvdb, dev, args):
         of, ob = pack._order_for(True, 128, dt), pack._order_for(False, 128, dt)
         fwd = lambda: ext.implicit_gemm(x, a_f, pack._map_fwd(), N, of)  # noqa: E731
         dgr = lambda: ext.implicit_gemm(R, kio, pack._map_bwd(), N, ob)  # noqa: E731
+        chunk = wgrad_chunk(P, dt)
         wgr = lambda: ext.wgrad(x, R, pairs, sizes, False, chunk)  # noqa: E731
         t = {"fwd_ms": _median_ms(torch, fwd, 10), "dgrad_ms": _median_ms(torch, dgr, 10),
              "wgrad_ms": _median_ms(torch, wgr, 10)}

@@ -34,9 +34,13 @@ def kio_to_weight(gk: torch.Tensor, kernel_size, transposed: bool) -> torch.Tens
     return g.permute(4, 3, 2, 1, 0) if not transposed else g.permute(3, 4, 2, 1, 0)
 
 
-def wgrad_chunk(n_pairs: int) -> int:
-    """Pairs per split-K chunk: a few thousand chunks per launch, multiples of 64."""
-    c = -(-max(n_pairs, 1) // 4096)
+def wgrad_chunk(n_pairs: int, dtype: torch.dtype = torch.float32) -> int:
+    """Pairs per split-K chunk, a multiple of 64 in [256, 16384].  fp32 (MFMA-bound) runs
+    best with ~4096 chunks per launch; 16-bit inputs with ~768, since their contraction is
+    fast and every chunk writes a [Cin, Cout] fp32 partial that the reduction reads back
+    (measured on MI355X: 6.7M pairs, 128 channels, bf16 0.94 -> 0.73 ms)."""
+    target = 768 if dtype in (torch.bfloat16, torch.float16) else 4096
+    c = -(-max(n_pairs, 1) // target)
     c = -(-c // 64) * 64
     return int(min(16384, max(256, c)))
 
@@ -110,7 +114,8 @@ class SparseConvFunction(torch.autograd.Function):
             gin = ext.implicit_gemm(grad_out, kio, dmap, n_in, order)
         if ctx.needs_input_grad[1]:
             pairs, sizes = pack._pairs()
-            gk = ext.wgrad(feats.contiguous(), grad_out, pairs, sizes, transposed, wgrad_chunk(pairs.shape[0]))
+            gk = ext.wgrad(feats.contiguous(), grad_out, pairs, sizes, transposed,
+                           wgrad_chunk(pairs.shape[0], feats.dtype))
             gw = kio_to_weight(gk.to(kio.dtype), pack.kernel_size, transposed)
         return gin, gw, gb, None, None, None
 
