@@ -473,14 +473,13 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
     int32_t* __restrict__ out = map + vbase * K;
     const int qstep = 64 / K, rstep = 64 % K;
     int r = lane / K, k = lane - (lane / K) * K;
-    for (int idx = lane; idx < total; idx += 64) {
-        const int bit = s_bits[r];
+    // probe coordinate of row bit `bit` for tap offsets `of`; false when the division is inexact
+    auto probe_q = [&](int bit, const int (&of)[3], int32_t (&q)[3]) -> bool {
         const int c[3] = {o[0] + (bit >> 6), o[1] + ((bit >> 3) & 7), o[2] + (bit & 7)};
-        int32_t q[3];
         bool ok = true;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const int num = c[a] * g.mul[a] + s_off[3 * k + a];
+            const int num = c[a] * g.mul[a] + of[a];
             if (g.shift[a] >= 0) {
                 if (num & ((1 << g.shift[a]) - 1)) ok = false;
                 q[a] = num >> g.shift[a];
@@ -489,18 +488,71 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
                 q[a] = num / g.div[a];
             }
         }
-        int32_t res = -1;
-        if (ok) {
-            if (staged) {
+        return ok;
+    };
+    if (staged) {
+        // U map entries per lane per pass with every LDS read unconditional (clamped
+        // indices): the reads of the U entries overlap, where one entry at a time made a
+        // chain of three dependent LDS round trips (voxel bit, mask word, word base).
+        constexpr int U = 4;
+        const int last_r = cnt - 1;
+        for (int idx0 = lane; idx0 < total; idx0 += 64 * U) {
+            int ru[U], ku[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ru[u] = r < last_r ? r : last_r;
+                ku[u] = k;
+                r += qstep;
+                k += rstep;
+                if (k >= K) {
+                    k -= K;
+                    ++r;
+                }
+            }
+            int bit[U], of[U][3];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                bit[u] = s_bits[ru[u]];
+                of[u][0] = s_off[3 * ku[u] + 0];
+                of[u][1] = s_off[3 * ku[u] + 1];
+                of[u][2] = s_off[3 * ku[u] + 2];
+            }
+            int word[U], qb[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                int32_t q[3];
+                ok[u] = probe_q(bit[u], of[u], q);
                 const int n = (((q[0] >> 3) - lo[0]) * ext[1] + ((q[1] >> 3) - lo[1])) * ext[2] + ((q[2] >> 3) - lo[2]);
-                const int qb = ((q[0] & 7) << 6) | ((q[1] & 7) << 3) | (q[2] & 7);
-                const uint64_t m = s_mask[n * 8 + (qb >> 6)];
-                if ((m >> (qb & 63)) & 1ull)
-                    res = s_base[n * 8 + (qb >> 6)] + __popcll(m & ((1ull << (qb & 63)) - 1ull));
-            } else if (probe.n_leaves > 0) {
-                res = voxel_index(probe, b, q[0], q[1], q[2]);
+                qb[u] = ((q[0] & 7) << 6) | ((q[1] & 7) << 3) | (q[2] & 7);
+                word[u] = ok[u] ? n * 8 + (qb[u] >> 6) : 0;
+            }
+            uint64_t m[U];
+            int32_t wb[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                m[u] = s_mask[word[u]];
+                wb[u] = s_base[word[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = idx0 + 64 * u;
+                if (idx < total) {
+                    const int sh = qb[u] & 63;
+                    const bool hit = ok[u] && ((m[u] >> sh) & 1ull);
+                    out[idx] = hit ? wb[u] + __popcll(m[u] & ((1ull << sh) - 1ull)) : -1;
+                }
             }
         }
+        return;
+    }
+    for (int idx = lane; idx < total; idx += 64) {
+        const int bit = s_bits[r];
+        const int of[3] = {s_off[3 * k + 0], s_off[3 * k + 1], s_off[3 * k + 2]};
+        int32_t q[3];
+        const bool ok = probe_q(bit, of, q);
+        int32_t res = -1;
+        if (ok && probe.n_leaves > 0) res = voxel_index(probe, b, q[0], q[1], q[2]);
         out[idx] = res;
         r += qstep;
         k += rstep;
