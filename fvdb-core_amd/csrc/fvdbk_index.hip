@@ -386,6 +386,7 @@ __device__ inline int64_t xcd_swizzle(int64_t bid, int64_t n) {
 // (1..512 voxels), and a workgroup holds its slots until its largest leaf is done.
 constexpr int KMAP_WAVES = 1;
 constexpr int KMAP_STAGE_BYTES = 16384; // LDS per wavefront for the staged neighbourhood
+constexpr int KMAP_STAGE_PER_LEAF = 128; // staged bytes per probe leaf: 16 mask words + 16 word bases
 
 // One wavefront per row-grid leaf.  The probe-grid leaves
 // that the leaf's taps can reach form a small box (nb_cap leaves at most, fixed per
@@ -394,19 +395,25 @@ constexpr int KMAP_STAGE_BYTES = 16384; // LDS per wavefront for the staged neig
 // ballot compaction of its mask (bit position per voxel rank, no ijk reads), and the
 // leaf's block of the map (cnt*K consecutive int32) is written by consecutive lanes.
 // With nb_cap == 0 (box too large to stage) every pair does a hash probe instead.
+// UNIT: every axis has mul = div = 1 (forward maps of stride-1 convolutions), so a probe
+// coordinate is the row coordinate plus the tap offset and every division is exact.
+template <bool UNIT>
 __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk_grid probe, ProbeGeom g, int nb_cap,
                                               int32_t* __restrict__ map) {
     extern __shared__ __attribute__((aligned(16))) unsigned char kmap_smem[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t blk = xcd_swizzle(blockIdx.x, g.n_blocks);
     const int64_t L = blk < 0 ? rows.n_leaves : blk * KMAP_WAVES + wave;
-    // LDS slice of this wavefront: masks [nb_cap][8] u64, bases [nb_cap][8] i32, bits [512] u16
-    unsigned char* base_ptr = kmap_smem + (size_t)wave * (nb_cap * 96 + 1024);
-    uint64_t* s_mask = reinterpret_cast<uint64_t*>(base_ptr);
+    // LDS slice of this wavefront: the staged leaves' masks as 32-bit words [nb_cap][16] and
+    // the index of each word's first active voxel [nb_cap][16] (32-bit words keep the
+    // lookup's shifts and popcounts 32-bit), then the leaf's voxel bits [512] u16
+    unsigned char* base_ptr = kmap_smem + (size_t)wave * (nb_cap * KMAP_STAGE_PER_LEAF + 1024);
+    uint32_t* s_mask = reinterpret_cast<uint32_t*>(base_ptr);
     int32_t* s_base = reinterpret_cast<int32_t*>(base_ptr + nb_cap * 64);
-    uint16_t* s_bits = reinterpret_cast<uint16_t*>(base_ptr + nb_cap * 96);
+    uint16_t* s_bits = reinterpret_cast<uint16_t*>(base_ptr + nb_cap * KMAP_STAGE_PER_LEAF);
     // per-tap offsets (x, y, z), shared by the workgroup
-    int32_t* s_off = reinterpret_cast<int32_t*>(kmap_smem + (size_t)KMAP_WAVES * (nb_cap * 96 + 1024));
+    int32_t* s_off =
+        reinterpret_cast<int32_t*>(kmap_smem + (size_t)KMAP_WAVES * (nb_cap * KMAP_STAGE_PER_LEAF + 1024));
     for (int k = threadIdx.x; k < g.K; k += 64 * KMAP_WAVES) {
         const int tz = k % g.ext[2], ty = (k / g.ext[2]) % g.ext[1], tx = k / (g.ext[2] * g.ext[1]);
         s_off[3 * k + 0] = g.off0[0] + g.sgn * tx;
@@ -459,8 +466,11 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
                 }
 #pragma unroll
                 for (int w = 0; w < 8; ++w) {
-                    s_mask[n * 8 + w] = mw[w];
-                    s_base[n * 8 + w] = bw[w];
+                    const uint32_t lo32 = (uint32_t)mw[w], hi32 = (uint32_t)(mw[w] >> 32);
+                    s_mask[n * 16 + 2 * w] = lo32;
+                    s_mask[n * 16 + 2 * w + 1] = hi32;
+                    s_base[n * 16 + 2 * w] = bw[w];
+                    s_base[n * 16 + 2 * w + 1] = bw[w] + __popc(lo32);
                 }
             }
         }
@@ -471,11 +481,14 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
     const int K = g.K;
     const int total = cnt * K;
     int32_t* __restrict__ out = map + vbase * K;
-    const int qstep = 64 / K, rstep = 64 % K;
-    int r = lane / K, k = lane - (lane / K) * K;
     // probe coordinate of row bit `bit` for tap offsets `of`; false when the division is inexact
     auto probe_q = [&](int bit, const int (&of)[3], int32_t (&q)[3]) -> bool {
         const int c[3] = {o[0] + (bit >> 6), o[1] + ((bit >> 3) & 7), o[2] + (bit & 7)};
+        if constexpr (UNIT) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) q[a] = c[a] + of[a];
+            return true;
+        }
         bool ok = true;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -490,44 +503,43 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
         }
         return ok;
     };
-    if (staged) {
-        // U map entries per lane per pass with every LDS read unconditional (clamped
-        // indices): the reads of the U entries overlap, where one entry at a time made a
-        // chain of three dependent LDS round trips (voxel bit, mask word, word base).
+    // staged lookup: 32-bit mask word of the probe voxel, then its index or -1
+    auto staged_index = [&](bool ok, const int32_t (&q)[3], uint32_t m, int32_t wb, int sh) -> int32_t {
+        (void)q;
+        return (ok && ((m >> sh) & 1u)) ? wb + __popc(m & ((1u << sh) - 1u)) : -1;
+    };
+    auto staged_word = [&](const int32_t (&q)[3], int& sh) -> int {
+        const int n = (((q[0] >> 3) - lo[0]) * ext[1] + ((q[1] >> 3) - lo[1])) * ext[2] + ((q[2] >> 3) - lo[2]);
+        const int qb = ((q[0] & 7) << 6) | ((q[1] & 7) << 3) | (q[2] & 7);
+        sh = qb & 31;
+        return n * 16 + (qb >> 5);
+    };
+    if (staged && K <= 64) {
+        // each lane keeps one tap (offsets in registers); a pass covers U groups of 64/K
+        // consecutive rows with every LDS read unconditional (clamped), so the reads of
+        // the U groups overlap
         constexpr int U = 4;
+        const int rpi = 64 / K, lk = lane % K, lr = lane / K;
+        const bool act = lane < rpi * K;
+        const int of[3] = {s_off[3 * lk + 0], s_off[3 * lk + 1], s_off[3 * lk + 2]};
         const int last_r = cnt - 1;
-        for (int idx0 = lane; idx0 < total; idx0 += 64 * U) {
-            int ru[U], ku[U];
+        for (int r0 = lr; r0 < cnt; r0 += rpi * U) {
+            int bit[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                ru[u] = r < last_r ? r : last_r;
-                ku[u] = k;
-                r += qstep;
-                k += rstep;
-                if (k >= K) {
-                    k -= K;
-                    ++r;
-                }
+                const int rr = r0 + rpi * u;
+                bit[u] = s_bits[rr < last_r ? rr : last_r];
             }
-            int bit[U], of[U][3];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                bit[u] = s_bits[ru[u]];
-                of[u][0] = s_off[3 * ku[u] + 0];
-                of[u][1] = s_off[3 * ku[u] + 1];
-                of[u][2] = s_off[3 * ku[u] + 2];
-            }
-            int word[U], qb[U];
+            int32_t q[U][3];
+            int word[U], sh[U];
             bool ok[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                int32_t q[3];
-                ok[u] = probe_q(bit[u], of[u], q);
-                const int n = (((q[0] >> 3) - lo[0]) * ext[1] + ((q[1] >> 3) - lo[1])) * ext[2] + ((q[2] >> 3) - lo[2]);
-                qb[u] = ((q[0] & 7) << 6) | ((q[1] & 7) << 3) | (q[2] & 7);
-                word[u] = ok[u] ? n * 8 + (qb[u] >> 6) : 0;
+                ok[u] = probe_q(bit[u], of, q[u]);
+                word[u] = ok[u] ? staged_word(q[u], sh[u]) : 0;
+                if (!ok[u]) sh[u] = 0;
             }
-            uint64_t m[U];
+            uint32_t m[U];
             int32_t wb[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -536,23 +548,27 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int idx = idx0 + 64 * u;
-                if (idx < total) {
-                    const int sh = qb[u] & 63;
-                    const bool hit = ok[u] && ((m[u] >> sh) & 1ull);
-                    out[idx] = hit ? wb[u] + __popcll(m[u] & ((1ull << sh) - 1ull)) : -1;
-                }
+                const int rr = r0 + rpi * u;
+                if (act && rr < cnt) out[(int64_t)rr * K + lk] = staged_index(ok[u], q[u], m[u], wb[u], sh[u]);
             }
         }
         return;
     }
+    const int qstep = 64 / K, rstep = 64 % K;
+    int r = lane / K, k = lane - (lane / K) * K;
     for (int idx = lane; idx < total; idx += 64) {
         const int bit = s_bits[r];
         const int of[3] = {s_off[3 * k + 0], s_off[3 * k + 1], s_off[3 * k + 2]};
         int32_t q[3];
         const bool ok = probe_q(bit, of, q);
         int32_t res = -1;
-        if (ok && probe.n_leaves > 0) res = voxel_index(probe, b, q[0], q[1], q[2]);
+        if (staged) {
+            int sh = 0;
+            const int word = ok ? staged_word(q, sh) : 0;
+            res = staged_index(ok, q, s_mask[word], s_base[word], sh);
+        } else if (ok && probe.n_leaves > 0) {
+            res = voxel_index(probe, b, q[0], q[1], q[2]);
+        }
         out[idx] = res;
         r += qstep;
         k += rstep;
@@ -921,13 +937,20 @@ static int kmap_launch(const fvdbk_grid* rows, const fvdbk_grid* probe, ProbeGeo
     }
     if (rows->n_leaves > 0) {
         int nb = kmap_box_bound(g);
-        if (nb * 96 + 1024 > KMAP_STAGE_BYTES) nb = 0;  // too large to stage: probe the hash per pair
-        const size_t lds = (size_t)KMAP_WAVES * (nb * 96 + 1024) + (size_t)K * 12;
+        if (nb * KMAP_STAGE_PER_LEAF + 1024 > KMAP_STAGE_BYTES) nb = 0;  // too large to stage: probe the hash per pair
+        const size_t lds = (size_t)KMAP_WAVES * (nb * KMAP_STAGE_PER_LEAF + 1024) + (size_t)K * 12;
         FVDBK_REQUIRE(lds <= 65536, FVDBK_EINVAL, "kernel volume %lld too large for the kernel-map launch",
                       (long long)K);
         g.n_blocks = (int)((rows->n_leaves + KMAP_WAVES - 1) / KMAP_WAVES);
         const unsigned blocks = (unsigned)((g.n_blocks + 7) / 8 * 8);
-        hipLaunchKernelGGL(k_kmap, dim3(blocks), dim3(64 * KMAP_WAVES), lds, stream, *rows, *probe, g, nb, map);
+        const bool unit = g.mul[0] == 1 && g.mul[1] == 1 && g.mul[2] == 1 && g.div[0] == 1 && g.div[1] == 1 &&
+                          g.div[2] == 1;
+        if (unit)
+            hipLaunchKernelGGL(k_kmap<true>, dim3(blocks), dim3(64 * KMAP_WAVES), lds, stream, *rows, *probe, g, nb,
+                               map);
+        else
+            hipLaunchKernelGGL(k_kmap<false>, dim3(blocks), dim3(64 * KMAP_WAVES), lds, stream, *rows, *probe, g, nb,
+                               map);
         FVDBK_LAUNCH_CHECK("k_kmap");
     }
     const int64_t tail = (n_rows_padded - rows->n_voxels) * K;
