@@ -240,7 +240,7 @@ def main() -> int:
     flops_a = 2.0 * P_a * 64 * 64
 
     # dominant kernel: conv A implicit GEMM (bf16), maps prebuilt
-    kio = fvdb._autograd.weight_to_kio(wab, False).transpose(1, 2).contiguous()
+    kio = fvdb._autograd.weight_to_koi(wab, False)
     fmap = pa._map_fwd()
     conv_a = lambda: fvdb.ext.implicit_gemm(xb, kio, fmap, N)  # noqa: E731
     if args.kernel_only:
@@ -280,7 +280,7 @@ def main() -> int:
 
     log("[bench] timing dominant kernel with HIP events")
     k_ms, _ = event_avg_ms(torch, conv_a, max(10, args.steps))
-    kio32 = fvdb._autograd.weight_to_kio(wa, False).transpose(1, 2).contiguous()
+    kio32 = fvdb._autograd.weight_to_koi(wa, False)
     k32_ms, _ = event_avg_ms(torch, lambda: fvdb.ext.implicit_gemm(x, kio32, fmap, N), max(10, args.steps))
 
     # algorithmic bytes of conv A (bf16): input rows + output rows + map + weights

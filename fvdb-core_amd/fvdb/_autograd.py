@@ -29,6 +29,13 @@ def weight_to_kio(weight: torch.Tensor, transposed: bool) -> torch.Tensor:
     return w.reshape(-1, w.shape[3], w.shape[4]).contiguous()
 
 
+def weight_to_koi(weight: torch.Tensor, transposed: bool) -> torch.Tensor:
+    """The forward contraction's operand [K, O, I] (``weight_to_kio`` with the last two
+    dimensions swapped) in one permute."""
+    w = weight.permute(4, 3, 2, 0, 1) if not transposed else weight.permute(4, 3, 2, 1, 0)
+    return w.reshape(-1, w.shape[3], w.shape[4]).contiguous()
+
+
 def kio_to_weight(gk: torch.Tensor, kernel_size, transposed: bool) -> torch.Tensor:
     g = gk.reshape(int(kernel_size[2]), int(kernel_size[1]), int(kernel_size[0]), gk.shape[1], gk.shape[2])
     return g.permute(4, 3, 2, 1, 0) if not transposed else g.permute(3, 4, 2, 1, 0)
@@ -76,15 +83,15 @@ class SparseConvFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, feats, weight, bias, pack, transposed: bool, relu: bool = False):
         n_src, n_tgt = pack.source_grid.total_voxels, pack.target_grid.total_voxels
-        kio = weight_to_kio(weight, transposed)  # [K, I, O]
         cin, dt = feats.shape[1], feats.dtype
         if not transposed:
             fmap, order, n_out = pack._map_fwd(), pack._order_for(True, cin, dt), n_tgt
         else:
             fmap, order, n_out = pack._map_bwd(), pack._order_for(False, cin, dt), n_src
-        out = ext.implicit_gemm(feats.contiguous(), kio.transpose(1, 2).contiguous(), fmap, n_out, order, bias,
+        out = ext.implicit_gemm(feats.contiguous(), weight_to_koi(weight, transposed), fmap, n_out, order, bias,
                                 bool(relu))
-        ctx.save_for_backward(feats, kio, out if relu else None)
+        # the [K, I, O] operand of dgrad is built from the weight in backward, when needed
+        ctx.save_for_backward(feats, weight, out if relu else None)
         ctx.pack = pack
         ctx.transposed = transposed
         ctx.relu = bool(relu)
@@ -93,8 +100,9 @@ class SparseConvFunction(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_out):
-        feats, kio, out = ctx.saved_tensors
+        feats, weight, out = ctx.saved_tensors
         pack, transposed = ctx.pack, ctx.transposed
+        kio = weight_to_kio(weight, transposed)  # [K, I, O]
         grad_out = grad_out.contiguous()
         if grad_out.dtype != feats.dtype:
             grad_out = grad_out.to(feats.dtype)
