@@ -401,18 +401,19 @@ def test_sorted_rows_bit_exact(dtype, cin, cout):
 
 
 @pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
-@pytest.mark.parametrize("cout", [64, 192])
-def test_direct_lds_tile_bit_exact(dtype, cout):
+@pytest.mark.parametrize("cout,large", [(64, False), (192, False), (192, True)])
+def test_direct_lds_tile_bit_exact(dtype, cout, large):
     """The direct-to-LDS contraction (16-bit, 64 input channels, 27 taps, rows in tree order,
     outputs in whole 64-channel tiles) equals the tap-skipping kernel run over an identity row
     order bit for bit (same accumulation order; a skipped tap adds exact zeros), with and
     without the fused bias + ReLU, on a ragged last tile, and the fp64 oracle within the
-    16-bit bound."""
+    16-bit bound.  The large grid (>= 1024 tiles of 256 rows) takes the 8-wavefront form."""
     fvdb = _fvdb()
     from fvdb._lib import ext
     from fvdb.utils.synthetic import sphere_points
 
-    grid = fvdb.GridBatch.from_points(sphere_points(seed=5, n_points=60_000, device="cuda"), voxel_sizes=1 / 96)
+    n_points, vs = (1_500_000, 1 / 512) if large else (60_000, 1 / 96)
+    grid = fvdb.GridBatch.from_points(sphere_points(seed=5, n_points=n_points, device="cuda"), voxel_sizes=vs)
     pack, _ = grid.sparse_conv_kernel_map(3, 1)
     n = grid.total_voxels
     kmap = pack._map_fwd()
@@ -427,6 +428,9 @@ def test_direct_lds_tile_bit_exact(dtype, cout):
         y = ext.implicit_gemm(x, a, kmap, n, None, b, relu)
         y_skip = ext.implicit_gemm(x, a, kmap, n, ident, b, relu)
         assert torch.equal(y, y_skip), (b is not None, relu)
+    if large:
+        assert (n + 255) // 256 * (cout // 64) >= 1024, n
+        return
     ref = oracle.sparse_conv(x.double().cpu(), w.double().cpu(), kmap[:n].cpu().numpy(), n, 1)
     assert rel_err(ext.implicit_gemm(x, a, kmap, n), ref) < REL_TOL[dtype]
 
