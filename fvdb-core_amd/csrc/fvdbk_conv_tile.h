@@ -441,13 +441,14 @@ __device__ inline void wait_vm_lds(int n) {
     }
 }
 
-// Direct-to-LDS form of k_conv_tile_k for the most common narrow layer: 16-bit features,
-// 64 input channels, 27 taps, rows in tree order, outputs in whole 64-channel tiles.
+// Direct-to-LDS form of k_conv_tile_k for the most common narrow layers: 16-bit features,
+// 64 input channels, KT = 27 or 8 taps (3x3x3, 2x2x2), rows in tree order, outputs in
+// whole 64-channel tiles.
 // Wavefronts of the same shape (32 rows x 64 output channels) and MFMA order, so the
 // results are bit-identical to k_conv_tile_k, in workgroups of NW = 4 or 8 wavefronts
 // (128 or 256 rows); the memory pipeline differs:
 //  * weights: each tap's 64 x 64 tile is copied global -> LDS by the load unit
-//    (global_load_lds_dwordx4, 2 per lane), two taps ahead into a ring of three
+//    (global_load_lds_dwordx4, 8/NW per lane), two taps ahead into a ring of three
 //    buffers, with no registers and no LDS store instructions.  Rows are 128 bytes
 //    unpadded; the 16-byte segment s of row r sits at position s ^ (r & 7), so the
 //    MFMA operand reads of 8 consecutive rows hit distinct banks.  The buffers are
@@ -455,7 +456,7 @@ __device__ inline void wait_vm_lds(int n) {
 //    buffer being read and does not wait for every outstanding load.
 //  * gathers: raw buffer loads over the feature rows; a missing neighbour takes an
 //    offset beyond the buffer, which loads zeros without a branch, so every tap issues
-//    exactly NI*2 gathers + 2 staging loads.  With that fixed count the end of each tap
+//    exactly NI*2 gathers + 8/NW staging loads.  With that fixed count the end of each tap
 //    waits (vmcnt) only for the loads the next tap needs, and the gathers run LX = 3
 //    taps ahead within 120 registers (4 waves per SIMD).
 //  * epilogue: the output tile goes through LDS (wavefront slices in the weight
@@ -467,7 +468,7 @@ __device__ inline void wait_vm_lds(int n) {
 // static __shared__ array inside a kernel template does not link).
 static __shared__ __attribute__((aligned(16))) uint4 dma_ring0[512], dma_ring1[512], dma_ring2[512];
 
-template <typename T, int LX, int NW>
+template <typename T, int LX, int NW, int KT>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict__ A, const int32_t* __restrict__ map,
                      int64_t n_rows, int M, int n_mt, int64_t n_tiles, T* __restrict__ Y, const T* __restrict__ bias,
@@ -475,14 +476,15 @@ void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict_
     static_assert(sizeof(T) == 2, "16-bit features");
     using F = Frag<T>;
     using V = typename F::vec;
-    constexpr int MI = 4, NI = 2, CIN = 64, KT = 27;
+    constexpr int MI = 4, NI = 2, CIN = 64;
     constexpr int RT = NW * NI * 16;            // rows per workgroup
     constexpr int WT = MI * 16 * CIN;          // elements of one tap tile (8 KB)
     constexpr int NCH = CIN / F::CH;
     constexpr int GPT = NI * NCH;              // gather loads per tap and lane
     constexpr int OUT_SLICE = NI * 16 * (MI * 16 + 8);
     static_assert(NW == 4 || NW == 8, "4 or 8 wavefronts");
-    static_assert(OUT_SLICE <= WT && (NW - 3) * OUT_SLICE <= RT * KT * 2, "output slices fit the weight and map buffers");
+    static_assert(KT > LX, "more taps than the gather lookahead");
+    static_assert(OUT_SLICE <= WT, "output slices fit the weight buffers");
     static_assert(WT * sizeof(T) == sizeof(dma_ring0), "ring buffer size");
     extern __shared__ __attribute__((aligned(16))) unsigned char tile_smem[];
     T* const sW0 = reinterpret_cast<T*>(dma_ring0);
@@ -588,25 +590,39 @@ void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict_
     store_tile_slice<T, MI, NI>(Y, acc, slice, lane, row0 + wave * NI * 16, nullptr, n_rows, M, m0, bias, relu);
 }
 
-// Launches k_conv_tile_dma when the layer fits it; false otherwise.  Large layers take
-// 8-wavefront workgroups of 256 rows, which stage each weight tile once for twice the rows
-// (conv A of the 2.3M-voxel sphere batch: -7..-12% on MI355X); smaller ones keep 128-row
-// workgroups so that the launch still fills the chip.
+// Launches k_conv_tile_dma when the layer fits it (K = 27 or 8); false otherwise.  Large
+// layers take 8-wavefront workgroups of 256 rows, which stage each weight tile once for
+// twice the rows (conv A of the 2.3M-voxel sphere batch: -7..-12% on MI355X); smaller
+// ones keep 128-row workgroups so that the launch still fills the chip.  Dynamic LDS
+// holds the map tile, and at the end the output slices of wavefronts 3.. (wavefronts
+// 0-2 use the weight ring).
 template <typename T>
-static bool launch_tile_dma(const T* X, int64_t n_in, const T* A, const int32_t* map, int64_t n_rows, int M, T* Y,
-                            const T* bias, int relu, hipStream_t s) {
-    if (M % 64 != 0 || n_in * 64 * (int64_t)sizeof(T) > 0x7FFFFF00 || ((uintptr_t)Y % 16) != 0) return false;
+static bool launch_tile_dma(const T* X, int64_t n_in, const T* A, const int32_t* map, int64_t n_rows, int K, int M,
+                            T* Y, const T* bias, int relu, hipStream_t s) {
+    if ((K != 27 && K != 8) || M % 64 != 0 || n_in * 64 * (int64_t)sizeof(T) > 0x7FFFFF00 ||
+        ((uintptr_t)Y % 16) != 0)
+        return false;
     const int n_mt = M / 64;
-    auto go = [&](auto nw) {
-        constexpr int NW = decltype(nw)::value;
+    auto go = [&](auto nw, auto kt) {
+        constexpr int NW = decltype(nw)::value, KT = decltype(kt)::value;
         const int64_t n_tiles = (n_rows + 32 * NW - 1) / (32 * NW) * n_mt;
         const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
-        const size_t lds = (size_t)32 * NW * 27 * 4;  // map tile (the weight ring is static LDS)
-        hipLaunchKernelGGL((k_conv_tile_dma<T, 3, NW>), grid, dim3(64 * NW), lds, s, X, n_in, A, map, n_rows, M, n_mt,
-                           n_tiles, Y, bias, relu);
+        const size_t map_bytes = (size_t)32 * NW * KT * 4;
+        const size_t out_bytes = (size_t)(NW - 3) * 2 * 16 * (64 + 8) * sizeof(T);
+        const size_t lds = map_bytes > out_bytes ? map_bytes : out_bytes;
+        hipLaunchKernelGGL((k_conv_tile_dma<T, 3, NW, KT>), grid, dim3(64 * NW), lds, s, X, n_in, A, map, n_rows, M,
+                           n_mt, n_tiles, Y, bias, relu);
     };
-    if ((n_rows + 255) / 256 * n_mt >= 1024) go(std::integral_constant<int, 8>{});
-    else go(std::integral_constant<int, 4>{});
+    using W4 = std::integral_constant<int, 4>;
+    using W8 = std::integral_constant<int, 8>;
+    const bool wide = (n_rows + 255) / 256 * n_mt >= 1024;
+    if (K == 27) {
+        if (wide) go(W8{}, std::integral_constant<int, 27>{});
+        else go(W4{}, std::integral_constant<int, 27>{});
+    } else {
+        if (wide) go(W8{}, std::integral_constant<int, 8>{});
+        else go(W4{}, std::integral_constant<int, 8>{});
+    }
     return true;
 }
 
@@ -872,7 +888,8 @@ static bool try_tile(const T* X, int64_t n_in, const T* A, const int32_t* map, c
             return true;
         }
         // tree-order rows, whole 64-channel output tiles: direct-to-LDS weights, deep gathers
-        if (K == 27 && perm == nullptr && launch_tile_dma<T>(X, n_in, A, map, n_rows, M, Y, bias, relu, s)) return true;
+        // (2x2x2 stride-2 64->128 of the sphere batch: 0.19 -> 0.16 ms on MI355X)
+        if (perm == nullptr && launch_tile_dma<T>(X, n_in, A, map, n_rows, K, M, Y, bias, relu, s)) return true;
     }
     if (M > 32 && tile_lds_bytes<T, 4, 2, CIN>(K) <= TILE_LDS_LIMIT) {
         launch_tile<T, 4, 2, CIN>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);

@@ -402,25 +402,26 @@ def test_sorted_rows_bit_exact(dtype, cin, cout):
 
 @pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
 @pytest.mark.parametrize("cout,large", [(64, False), (192, False), (192, True)])
-def test_direct_lds_tile_bit_exact(dtype, cout, large):
-    """The direct-to-LDS contraction (16-bit, 64 input channels, 27 taps, rows in tree order,
-    outputs in whole 64-channel tiles) equals the tap-skipping kernel run over an identity row
-    order bit for bit (same accumulation order; a skipped tap adds exact zeros), with and
-    without the fused bias + ReLU, on a ragged last tile, and the fp64 oracle within the
-    16-bit bound.  The large grid (>= 1024 tiles of 256 rows) takes the 8-wavefront form."""
+@pytest.mark.parametrize("ks,st", [(3, 1), (2, 2)])
+def test_direct_lds_tile_bit_exact(dtype, cout, large, ks, st):
+    """The direct-to-LDS contraction (16-bit, 64 input channels, 27 or 8 taps, rows in tree
+    order, outputs in whole 64-channel tiles) equals the tap-skipping kernel run over an
+    identity row order bit for bit (same accumulation order; a skipped tap adds exact zeros),
+    with and without the fused bias + ReLU, on a ragged last tile, and the fp64 oracle within
+    the 16-bit bound.  The large grid (>= 1024 tiles of 256 rows) takes the 8-wavefront form."""
     fvdb = _fvdb()
     from fvdb._lib import ext
     from fvdb.utils.synthetic import sphere_points
 
     n_points, vs = (1_500_000, 1 / 512) if large else (60_000, 1 / 96)
     grid = fvdb.GridBatch.from_points(sphere_points(seed=5, n_points=n_points, device="cuda"), voxel_sizes=vs)
-    pack, _ = grid.sparse_conv_kernel_map(3, 1)
-    n = grid.total_voxels
+    pack, tgt = grid.sparse_conv_kernel_map(ks, st)
+    n_in, n = grid.total_voxels, tgt.total_voxels
     kmap = pack._map_fwd()
-    print(f"[direct-lds] {dtype} cout={cout} rows={n} (last tile {n % 128} rows)", flush=True)
+    print(f"[direct-lds] {dtype} {ks}^3 s{st} cout={cout} rows={n} (last tile {n % 128} rows)", flush=True)
     g = torch.Generator().manual_seed(21)
-    x = torch.randn(n, 64, generator=g).to("cuda", dtype)
-    w = (0.05 * torch.randn(cout, 64, 3, 3, 3, generator=g)).to("cuda", dtype)
+    x = torch.randn(n_in, 64, generator=g).to("cuda", dtype)
+    w = (0.05 * torch.randn(cout, 64, ks, ks, ks, generator=g)).to("cuda", dtype)
     a = fvdb._autograd.weight_to_kio(w, False).transpose(1, 2).contiguous()  # [K, M, C]
     bias = torch.randn(cout, generator=g).to("cuda", dtype)
     ident = torch.arange(n, dtype=torch.int32, device="cuda")
@@ -431,7 +432,7 @@ def test_direct_lds_tile_bit_exact(dtype, cout, large):
     if large:
         assert (n + 255) // 256 * (cout // 64) >= 1024, n
         return
-    ref = oracle.sparse_conv(x.double().cpu(), w.double().cpu(), kmap[:n].cpu().numpy(), n, 1)
+    ref = oracle.sparse_conv(x.double().cpu(), w.double().cpu(), kmap[:n].cpu().numpy(), n, st)
     assert rel_err(ext.implicit_gemm(x, a, kmap, n), ref) < REL_TOL[dtype]
 
 
