@@ -76,46 +76,22 @@ int64_t next_pow2(int64_t x) {
 
 }  // namespace
 
-// Build a grid batch index from voxel coordinates (duplicates merged, entries with a
-// negative grid index skipped).  Two device->host reads: (leaf count, out-of-range
-// count) and the grid offsets.  Returns [ijk, jidx, grid offsets (host int64 [B+1]),
-// grid offsets (device), leaf_keys, leaf_masks, word_base, hash].
-std::vector<at::Tensor> build_grid(at::Tensor ijk, at::Tensor jidx, int64_t n_grids) {
-    require_hip(ijk, "ijk");
-    TORCH_CHECK_VALUE(ijk.dim() == 2 && ijk.size(1) == 3, "ijk must have shape [N, 3]");
-    TORCH_CHECK_TYPE(ijk.scalar_type() == at::kInt, "ijk must be int32");
-    TORCH_CHECK_VALUE(n_grids >= 0 && n_grids <= 1024, "a grid batch holds at most 1024 grids");
-    ijk = ijk.contiguous();
-    const bool has_jidx = jidx.numel() > 0;
-    if (has_jidx) {
-        TORCH_CHECK_VALUE(jidx.numel() == ijk.size(0), "jidx must have one entry per voxel");
-        jidx = jidx.to(at::kInt).contiguous();
-    }
-    c10::hip::HIPGuard guard(ijk.device().index());
-    hipStream_t s = stream_of(ijk);
-    const int64_t n = ijk.size(0);
-    auto o32 = ijk.options().dtype(at::kInt);
-    auto o64 = ijk.options().dtype(at::kLong);
+namespace {
 
-    // unique leaf keys (unsorted) and the out-of-range count: one device->host read
-    const int64_t set_cap = fvdbk_grid_dedupe_capacity(n);
-    at::Tensor set = at::empty({set_cap}, o64);
-    at::Tensor lkeys = at::empty({std::max<int64_t>(n, 1)}, o64);
-    at::Tensor counters = at::zeros({2}, o32);
-    check_rc(fvdbk_grid_leaf_dedupe(ijk.data_ptr<int32_t>(), has_jidx ? jidx.data_ptr<int32_t>() : nullptr, n,
-                                    (int32_t)n_grids, reinterpret_cast<uint64_t*>(set.data_ptr<int64_t>()), set_cap,
-                                    reinterpret_cast<uint64_t*>(lkeys.data_ptr<int64_t>()),
-                                    reinterpret_cast<uint32_t*>(counters.data_ptr<int32_t>()), s),
-             "grid_leaf_dedupe");
-    at::Tensor hc = counters.to(at::kCPU);
-    const int64_t L = hc.data_ptr<int32_t>()[0];
-    const int nbad = hc.data_ptr<int32_t>()[1];
-    TORCH_CHECK_VALUE(nbad == 0, nbad, " voxels have a grid index >= ", n_grids,
-                      " or coordinates outside [-2^19, 2^19)");
+// Steps after the leaf de-duplication, shared by the builds from voxels and from a fine
+// grid's leaves: sort the unique keys (tree order), hash them, fill the masks
+// (`fill_masks(hash, cap, masks, L)`), count, scan, grid offsets (one device->host
+// read) and finalize.  Returns [ijk, jidx, grid offsets (host int64 [B+1]), grid
+// offsets (device), leaf_keys, leaf_masks, word_base, hash].
+template <typename FillMasks>
+std::vector<at::Tensor> finish_grid(const at::Tensor& lkeys, int64_t L, int64_t n_grids, hipStream_t s,
+                                    FillMasks fill_masks) {
+    auto o32 = lkeys.options().dtype(at::kInt);
+    auto o64 = lkeys.options().dtype(at::kLong);
     // tree order of the leaves: radix-sort only the unique keys
     at::Tensor ukeys = at::empty({L}, o64);
     const size_t ws_bytes = fvdbk_grid_sort_leaf_keys_workspace(L, (int32_t)n_grids);
-    at::Tensor ws = at::empty({(int64_t)std::max<size_t>(ws_bytes, 1)}, ijk.options().dtype(at::kByte));
+    at::Tensor ws = at::empty({(int64_t)std::max<size_t>(ws_bytes, 1)}, lkeys.options().dtype(at::kByte));
     check_rc(fvdbk_grid_sort_leaf_keys(reinterpret_cast<const uint64_t*>(lkeys.data_ptr<int64_t>()),
                                        reinterpret_cast<uint64_t*>(ukeys.data_ptr<int64_t>()), L, (int32_t)n_grids,
                                        ws.data_ptr(), ws_bytes, s),
@@ -126,10 +102,8 @@ std::vector<at::Tensor> build_grid(at::Tensor ijk, at::Tensor jidx, int64_t n_gr
                                    reinterpret_cast<fvdbk_hslot*>(hash.data_ptr<int64_t>()), cap, s),
              "grid_hash_build");
     at::Tensor masks = at::empty({L, 8}, o64);
-    check_rc(fvdbk_grid_leaf_masks(ijk.data_ptr<int32_t>(), has_jidx ? jidx.data_ptr<int32_t>() : nullptr, n,
-                                   reinterpret_cast<const fvdbk_hslot*>(hash.data_ptr<int64_t>()), cap,
-                                   reinterpret_cast<uint64_t*>(masks.data_ptr<int64_t>()), L, s),
-             "grid_leaf_masks");
+    fill_masks(reinterpret_cast<const fvdbk_hslot*>(hash.data_ptr<int64_t>()), cap,
+               reinterpret_cast<uint64_t*>(masks.data_ptr<int64_t>()), L);
     at::Tensor counts = at::empty({L}, o32);
     check_rc(fvdbk_grid_leaf_counts(reinterpret_cast<const uint64_t*>(masks.data_ptr<int64_t>()), L,
                                     counts.data_ptr<int32_t>(), s),
@@ -152,6 +126,89 @@ std::vector<at::Tensor> build_grid(at::Tensor ijk, at::Tensor jidx, int64_t n_gr
                                  nullptr, (int32_t)n_grids, s),
              "grid_finalize");
     return {ijk_out, jidx_out, jo_host, jo, ukeys, masks, wbase, hash};
+}
+
+// Unique leaf keys (unsorted) of the given voxels; one device->host read (leaf count,
+// out-of-range count).
+std::pair<at::Tensor, int64_t> dedupe_leaves(const at::Tensor& ijk, const at::Tensor& jidx, bool has_jidx,
+                                             int64_t n_grids, hipStream_t s) {
+    const int64_t n = ijk.size(0);
+    auto o32 = ijk.options().dtype(at::kInt);
+    auto o64 = ijk.options().dtype(at::kLong);
+    const int64_t set_cap = fvdbk_grid_dedupe_capacity(n);
+    at::Tensor set = at::empty({set_cap}, o64);
+    at::Tensor lkeys = at::empty({std::max<int64_t>(n, 1)}, o64);
+    at::Tensor counters = at::zeros({2}, o32);
+    check_rc(fvdbk_grid_leaf_dedupe(ijk.data_ptr<int32_t>(), has_jidx ? jidx.data_ptr<int32_t>() : nullptr, n,
+                                    (int32_t)n_grids, reinterpret_cast<uint64_t*>(set.data_ptr<int64_t>()), set_cap,
+                                    reinterpret_cast<uint64_t*>(lkeys.data_ptr<int64_t>()),
+                                    reinterpret_cast<uint32_t*>(counters.data_ptr<int32_t>()), s),
+             "grid_leaf_dedupe");
+    at::Tensor hc = counters.to(at::kCPU);
+    const int64_t L = hc.data_ptr<int32_t>()[0];
+    const int nbad = hc.data_ptr<int32_t>()[1];
+    TORCH_CHECK_VALUE(nbad == 0, nbad, " voxels have a grid index >= ", n_grids,
+                      " or coordinates outside [-2^19, 2^19)");
+    return {lkeys, L};
+}
+
+}  // namespace
+
+// Build a grid batch index from voxel coordinates (duplicates merged, entries with a
+// negative grid index skipped).  Two device->host reads: (leaf count, out-of-range
+// count) and the grid offsets.  Returns [ijk, jidx, grid offsets (host int64 [B+1]),
+// grid offsets (device), leaf_keys, leaf_masks, word_base, hash].
+std::vector<at::Tensor> build_grid(at::Tensor ijk, at::Tensor jidx, int64_t n_grids) {
+    require_hip(ijk, "ijk");
+    TORCH_CHECK_VALUE(ijk.dim() == 2 && ijk.size(1) == 3, "ijk must have shape [N, 3]");
+    TORCH_CHECK_TYPE(ijk.scalar_type() == at::kInt, "ijk must be int32");
+    TORCH_CHECK_VALUE(n_grids >= 0 && n_grids <= 1024, "a grid batch holds at most 1024 grids");
+    ijk = ijk.contiguous();
+    const bool has_jidx = jidx.numel() > 0;
+    if (has_jidx) {
+        TORCH_CHECK_VALUE(jidx.numel() == ijk.size(0), "jidx must have one entry per voxel");
+        jidx = jidx.to(at::kInt).contiguous();
+    }
+    c10::hip::HIPGuard guard(ijk.device().index());
+    hipStream_t s = stream_of(ijk);
+    const int64_t n = ijk.size(0);
+    auto [lkeys, L] = dedupe_leaves(ijk, jidx, has_jidx, n_grids, s);
+    return finish_grid(lkeys, L, n_grids, s, [&](const fvdbk_hslot* hash, int64_t cap, uint64_t* masks, int64_t nl) {
+        check_rc(fvdbk_grid_leaf_masks(ijk.data_ptr<int32_t>(), has_jidx ? jidx.data_ptr<int32_t>() : nullptr, n,
+                                       hash, cap, masks, nl, s),
+                 "grid_leaf_masks");
+    });
+}
+
+// Grid of floor(ijk / factor) over a built grid's voxels, for power-of-two factors
+// (x, y, z), from the fine grid's leaf keys and masks: the de-duplication takes one
+// voxel per fine leaf and the coarse masks are ORed from the fine masks.  Same outputs
+// as build_grid over the coarsened voxels.
+std::vector<at::Tensor> coarsen_grid(at::Tensor fine_keys, at::Tensor fine_masks, int64_t n_grids,
+                                     std::vector<int64_t> factor) {
+    require_hip(fine_keys, "leaf keys");
+    auto f3 = vec3(factor, "coarsening_factor");
+    TORCH_CHECK_VALUE(fine_keys.dim() == 1 && fine_masks.dim() == 2 && fine_masks.size(0) == fine_keys.size(0) &&
+                          fine_masks.size(1) == 8,
+                      "leaf keys [L] and masks [L, 8] expected");
+    TORCH_CHECK_VALUE(n_grids >= 0 && n_grids <= 1024, "a grid batch holds at most 1024 grids");
+    fine_keys = fine_keys.contiguous();
+    fine_masks = fine_masks.contiguous();
+    c10::hip::HIPGuard guard(fine_keys.device().index());
+    hipStream_t s = stream_of(fine_keys);
+    const int64_t Lf = fine_keys.size(0);
+    at::Tensor rep = at::empty({Lf, 3}, fine_keys.options().dtype(at::kInt));
+    at::Tensor rep_b = at::empty({Lf}, fine_keys.options().dtype(at::kInt));
+    check_rc(fvdbk_grid_coarsen_reps(reinterpret_cast<const uint64_t*>(fine_keys.data_ptr<int64_t>()), Lf, f3.data(),
+                                     rep.data_ptr<int32_t>(), rep_b.data_ptr<int32_t>(), s),
+             "grid_coarsen_reps");
+    auto [lkeys, L] = dedupe_leaves(rep, rep_b, true, n_grids, s);
+    return finish_grid(lkeys, L, n_grids, s, [&](const fvdbk_hslot* hash, int64_t cap, uint64_t* masks, int64_t nl) {
+        check_rc(fvdbk_grid_coarsen_masks(reinterpret_cast<const uint64_t*>(fine_keys.data_ptr<int64_t>()),
+                                          reinterpret_cast<const uint64_t*>(fine_masks.data_ptr<int64_t>()), Lf,
+                                          f3.data(), hash, cap, masks, nl, s),
+                 "grid_coarsen_masks");
+    });
 }
 
 at::Tensor ijk_to_index(at::Tensor g_ijk, at::Tensor g_keys, at::Tensor g_masks, at::Tensor g_wbase, at::Tensor g_hash,
@@ -444,6 +501,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.doc() = "fvdb-core_amd tensor layer over the fvdbk C ABI";
     m.def("version", &version);
     m.def("build_grid", &build_grid);
+    m.def("coarsen_grid", &coarsen_grid);
     m.def("ijk_to_index", &ijk_to_index);
     m.def("points_to_ijk", &points_to_ijk);
     m.def("conv_grid_candidates", &conv_grid_candidates);

@@ -193,6 +193,52 @@ __global__ __launch_bounds__(256) void k_leaf_masks(const int32_t* __restrict__ 
     if (word >= 0 && tail) atomicOr((unsigned long long*)&masks[word], bits);
 }
 
+// Coarsening by power-of-two factors (1 << sh[a] per axis a = x, y, z) straight from a
+// fine grid's leaves instead of its voxels.  Every voxel of a fine leaf falls into one
+// coarse leaf: the fine leaf spans 8 aligned coordinates per axis, and 8 >> sh (or a
+// single coordinate) of them stay within an aligned block of 8 coarse coordinates.
+// Per fine leaf, one coarse voxel of that coarse leaf (the leaf origin shifted down),
+// for the leaf de-duplication.
+__global__ void k_coarse_reps(const uint64_t* __restrict__ fine_keys, int64_t n, int sx, int sy, int sz,
+                              int32_t* __restrict__ rep_ijk, int32_t* __restrict__ rep_bidx) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    int32_t b, x, y, z;
+    decode_leaf_key(fine_keys[l], b, x, y, z);
+    rep_ijk[3 * l + 0] = x >> sx;  // arithmetic shift: floor division
+    rep_ijk[3 * l + 1] = y >> sy;
+    rep_ijk[3 * l + 2] = z >> sz;
+    rep_bidx[l] = b;
+}
+
+// Coarse leaf masks from the fine masks: one thread per (fine leaf, x slab w).  The
+// slab's 64 bits (y, z) map to coarse bits (((y0 + y) >> sy) & 7, ((z0 + z) >> sz) & 7)
+// of coarse slab ((x0 + w) >> sx) & 7; the thread ORs them over its set bits and
+// issues one atomic OR (slabs that share a coarse slab merge there).
+__global__ void k_coarse_masks(const uint64_t* __restrict__ fine_keys, const uint64_t* __restrict__ fine_masks,
+                               int64_t n, int sx, int sy, int sz, const fvdbk_hslot* __restrict__ h, int64_t cap,
+                               uint64_t* __restrict__ coarse_masks) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * 8) return;
+    const int64_t l = t >> 3;
+    const int w = (int)(t & 7);
+    uint64_t m = fine_masks[l * 8 + w];
+    if (m == 0ull) return;
+    int32_t b, x0, y0, z0;
+    decode_leaf_key(fine_keys[l], b, x0, y0, z0);
+    const int32_t cx = (x0 + w) >> sx, cy0 = y0 >> sy, cz0 = z0 >> sz;
+    const int32_t leaf = hash_find(h, cap, leaf_key(b, cx, cy0, cz0));
+    if (leaf < 0) return;  // not reached: every coarse leaf was inserted
+    uint64_t acc = 0ull;
+    while (m) {
+        const int p = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1ull;
+        const int y = p >> 3, z = p & 7;
+        acc |= 1ull << (((((y0 + y) >> sy) & 7) << 3) | (((z0 + z) >> sz) & 7));
+    }
+    atomicOr((unsigned long long*)&coarse_masks[(int64_t)leaf * 8 + (cx & 7)], (unsigned long long)acc);
+}
+
 __global__ void k_leaf_counts(const uint64_t* __restrict__ masks, int64_t n_leaves, int32_t* __restrict__ counts) {
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n_leaves) return;
@@ -830,6 +876,44 @@ int fvdbk_grid_leaf_masks(const int32_t* ijk, const int32_t* batch_idx, int64_t 
     hipLaunchKernelGGL(k_leaf_masks, dim3(grid_1d(n, 256)), dim3(256), 0, stream, ijk, batch_idx, n, hash, hash_cap,
                        leaf_masks);
     FVDBK_LAUNCH_CHECK("k_leaf_masks");
+    return FVDBK_OK;
+}
+
+static int coarsen_shifts(const int32_t factor[3], int (&sh)[3]) {
+    for (int a = 0; a < 3; ++a) {
+        const int32_t f = factor[a];
+        FVDBK_REQUIRE(f > 0 && f <= (1 << 16) && (f & (f - 1)) == 0, FVDBK_EINVAL,
+                      "coarsening factor %d is not a power of two in [1, 65536]", (int)f);
+        sh[a] = __builtin_ctz((unsigned)f);
+    }
+    return FVDBK_OK;
+}
+
+int fvdbk_grid_coarsen_reps(const uint64_t* fine_keys, int64_t n_fine_leaves, const int32_t factor[3],
+                            int32_t* rep_ijk, int32_t* rep_bidx, hipStream_t stream) {
+    int sh[3];
+    const int rc = coarsen_shifts(factor, sh);
+    if (rc != FVDBK_OK) return rc;
+    if (n_fine_leaves == 0) return FVDBK_OK;
+    hipLaunchKernelGGL(k_coarse_reps, dim3(grid_1d(n_fine_leaves, 256)), dim3(256), 0, stream, fine_keys,
+                       n_fine_leaves, sh[0], sh[1], sh[2], rep_ijk, rep_bidx);
+    FVDBK_LAUNCH_CHECK("k_coarse_reps");
+    return FVDBK_OK;
+}
+
+int fvdbk_grid_coarsen_masks(const uint64_t* fine_keys, const uint64_t* fine_masks, int64_t n_fine_leaves,
+                             const int32_t factor[3], const fvdbk_hslot* hash, int64_t hash_cap, uint64_t* leaf_masks,
+                             int64_t n_leaves, hipStream_t stream) {
+    int sh[3];
+    const int rc = coarsen_shifts(factor, sh);
+    if (rc != FVDBK_OK) return rc;
+    if (n_leaves == 0) return FVDBK_OK;
+    hipLaunchKernelGGL(k_zero_u64, dim3(grid_1d(n_leaves * 8, 256)), dim3(256), 0, stream, leaf_masks, n_leaves * 8);
+    FVDBK_LAUNCH_CHECK("k_zero_u64");
+    if (n_fine_leaves == 0) return FVDBK_OK;
+    hipLaunchKernelGGL(k_coarse_masks, dim3(grid_1d(n_fine_leaves * 8, 256)), dim3(256), 0, stream, fine_keys,
+                       fine_masks, n_fine_leaves, sh[0], sh[1], sh[2], hash, hash_cap, leaf_masks);
+    FVDBK_LAUNCH_CHECK("k_coarse_masks");
     return FVDBK_OK;
 }
 

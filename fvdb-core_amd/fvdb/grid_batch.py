@@ -290,17 +290,34 @@ class GridBatch:
         st = to_vec3i(stride, "stride", non_negative=True)
         if any(s <= 0 for s in st):
             raise ValueError("stride must be strictly positive.")
+        if (st == (1, 1, 1) or st == ks) and self._coarsen_from_leaves(st):
+            return self._coarsened_from_leaves(st)
         cijk, cj = ext.conv_grid_candidates(self._impl.ijk, self._impl.jidx, list(ks), list(st))
         return self._derived(cijk, cj)
+
+    def _coarsen_from_leaves(self, factor) -> bool:
+        """Power-of-two factors on a non-empty grid: the coarse grid is built from the
+        leaves (one de-duplication entry per fine leaf, masks ORed from the fine masks)."""
+        return self.total_voxels > 0 and all(f > 0 and f & (f - 1) == 0 and f <= 65536 for f in factor)
+
+    def _coarsened_from_leaves(self, factor, voxel_sizes=None, origins=None) -> "GridBatch":
+        ijk_o, jidx_o, jo_host, jo, keys, masks, wbase, hash_ = ext.coarsen_grid(
+            self._impl.keys, self._impl.masks, self.grid_count, list(factor))
+        vs = self._impl.voxel_sizes if voxel_sizes is None else voxel_sizes
+        og = self._impl.origins if origins is None else origins
+        return GridBatch(impl=_GridIndex(ijk_o, jidx_o, jo_host, jo, keys, masks, wbase, hash_, self.grid_count,
+                                         vs.clone(), og.clone()))
 
     def coarsened_grid(self, coarsening_factor) -> "GridBatch":
         """Voxels {floor(ijk / factor)} with voxel size scaled by the factor."""
         f = to_vec3i(coarsening_factor, "coarsening_factor", positive=True)
-        # coarse mode: floor(ijk / factor) with the factor indexed by coordinate axis
-        cijk, cj = ext.conv_grid_candidates(self._impl.ijk, self._impl.jidx, list(f), list(f))
         fac = torch.tensor(f, dtype=torch.float64)
         vs = self._impl.voxel_sizes * fac
         og = self._impl.origins + 0.5 * (fac - 1) * self._impl.voxel_sizes
+        if self._coarsen_from_leaves(f):
+            return self._coarsened_from_leaves(f, vs, og)
+        # coarse mode: floor(ijk / factor) with the factor indexed by coordinate axis
+        cijk, cj = ext.conv_grid_candidates(self._impl.ijk, self._impl.jidx, list(f), list(f))
         return self._derived(cijk, cj, vs, og)
 
     def subdivided_grid(self, subdiv_factor, mask: Optional[JaggedTensor] = None) -> "GridBatch":

@@ -128,6 +128,23 @@ int fvdbk_grid_hash_build(const uint64_t* leaf_keys, int64_t n_leaves, fvdbk_hsl
 int fvdbk_grid_leaf_masks(const int32_t* ijk, const int32_t* batch_idx, int64_t n, const fvdbk_hslot* hash,
                           int64_t hash_cap, uint64_t* leaf_masks, int64_t n_leaves, hipStream_t stream);
 
+/* Coarsening by power-of-two factors (x, y, z), from the fine grid's leaves instead of
+ * its voxels: the coarse grid holds floor(ijk / factor) of every fine voxel, as
+ * dispatchCoarseIJKForFineGrid + dispatchCreateNanoGridFromIJK build it
+ * (src/fvdb/detail/ops/BuildCoarseGridFromFine.cu:24-29, CoarseIjkForFineGrid.cu:34;
+ * conv target grids with stride == kernel size, BuildGridForConv.cu:133-134).
+ *   fvdbk_grid_coarsen_reps   one coarse voxel per fine leaf (rep_ijk [n][3], rep_bidx [n]):
+ *                             the input of step 1 / the leaf de-duplication in place of
+ *                             the voxels (every voxel of a fine leaf lands in that coarse leaf)
+ *   fvdbk_grid_coarsen_masks  step 4 for the coarse leaves: their masks from the fine
+ *                             masks (leaf_masks [n_leaves][8] is cleared by the call)
+ * Factors must be powers of two in [1, 65536] (FVDBK_EINVAL otherwise). */
+int fvdbk_grid_coarsen_reps(const uint64_t* fine_keys, int64_t n_fine_leaves, const int32_t factor[3],
+                            int32_t* rep_ijk, int32_t* rep_bidx, hipStream_t stream);
+int fvdbk_grid_coarsen_masks(const uint64_t* fine_keys, const uint64_t* fine_masks, int64_t n_fine_leaves,
+                             const int32_t factor[3], const fvdbk_hslot* hash, int64_t hash_cap, uint64_t* leaf_masks,
+                             int64_t n_leaves, hipStream_t stream);
+
 int fvdbk_grid_leaf_counts(const uint64_t* leaf_masks, int64_t n_leaves, int32_t* counts, hipStream_t stream);
 
 /* Writes leaf_word_base [n_leaves][8], ijk_out [n_voxels][3], jidx_out [n_voxels]

@@ -160,3 +160,37 @@ def test_modules_chain():
     v = fvdb.nn.VDBTensor(sparse, sparse.jagged_like(torch.ones(2, 1, device="cuda")))
     out = fvdb.nn.MaxPool(2)(v, ref)
     assert out.data.jdata.reshape(-1).tolist() == [1.0, 0.0, 1.0]
+
+
+@pytest.mark.parametrize("factor", [(2, 2, 2), (1, 2, 4), (4, 1, 2), (8, 8, 8), (1, 1, 1), (32, 16, 2)])
+def test_coarsen_from_leaves_bit_exact(factor):
+    """Coarsening by power-of-two factors is built from the fine grid's leaves (one
+    de-duplication entry per leaf, coarse masks ORed from the fine masks).  Every output
+    of the index (voxels in tree order, grid offsets, leaf keys, masks, word bases) equals
+    the build from the coarsened voxels bit for bit, and the voxels equal the oracle's
+    floor(ijk / factor) set in tree order, on grids with negative coordinates, several
+    grids, and one of 700k voxels on sphere shells."""
+    fvdb = _fvdb()
+    from fvdb._lib import ext
+    from fvdb.utils.synthetic import sphere_points
+
+    rng = np.random.default_rng(sum(factor))
+    small, small_np = grids(fvdb, rng, [900, 1, 400], 60)
+    big = fvdb.GridBatch.from_points(sphere_points(seed=9, n_points=700_000, device="cuda"), voxel_sizes=1 / 300)
+    for g, per in ((small, small_np), (big, None)):
+        print(f"[case] coarsen factor={factor} voxels={g.total_voxels}", flush=True)
+        fast = g.coarsened_grid(factor)
+        cijk, cj = ext.conv_grid_candidates(g._impl.ijk, g._impl.jidx, list(factor), list(factor))
+        ref = g._derived(cijk, cj)
+        a, b = fast._impl, ref._impl
+        for name in ("ijk", "jidx", "joffsets", "keys", "masks", "wbase"):
+            assert torch.equal(getattr(a, name), getattr(b, name)), name
+        if per is not None:
+            offs = fast.joffsets.cpu().tolist()
+            for i, p in enumerate(per):
+                want = oracle.sort_unique(np.floor_divide(p, np.asarray(factor, dtype=np.int32)))
+                assert np.array_equal(fast.ijk.jdata[offs[i]:offs[i + 1]].cpu().numpy(), want)
+    # conv target grid with stride == kernel size takes the same path
+    t = small.conv_grid(2, 2)
+    want = small.coarsened_grid(2)
+    assert torch.equal(t.ijk.jdata, want.ijk.jdata) and torch.equal(t._impl.masks, want._impl.masks)
