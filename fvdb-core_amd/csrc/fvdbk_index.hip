@@ -432,7 +432,8 @@ __device__ inline int64_t xcd_swizzle(int64_t bid, int64_t n) {
 // (1..512 voxels), and a workgroup holds its slots until its largest leaf is done.
 constexpr int KMAP_WAVES = 1;
 constexpr int KMAP_STAGE_BYTES = 16384; // LDS per wavefront for the staged neighbourhood
-constexpr int KMAP_STAGE_PER_LEAF = 128; // staged bytes per probe leaf: 16 mask words + 16 word bases
+constexpr int KMAP_STAGE_PER_LEAF = 128; // staged bytes per probe leaf: 16 (mask word, word base) pairs
+constexpr int KMAP_ROW_BYTES = 2048;     // the row leaf's voxels: 512 packed local coordinates
 
 // One wavefront per row-grid leaf.  The probe-grid leaves
 // that the leaf's taps can reach form a small box (nb_cap leaves at most, fixed per
@@ -450,16 +451,16 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t blk = xcd_swizzle(blockIdx.x, g.n_blocks);
     const int64_t L = blk < 0 ? rows.n_leaves : blk * KMAP_WAVES + wave;
-    // LDS slice of this wavefront: the staged leaves' masks as 32-bit words [nb_cap][16] and
-    // the index of each word's first active voxel [nb_cap][16] (32-bit words keep the
-    // lookup's shifts and popcounts 32-bit), then the leaf's voxel bits [512] u16
-    unsigned char* base_ptr = kmap_smem + (size_t)wave * (nb_cap * KMAP_STAGE_PER_LEAF + 1024);
-    uint32_t* s_mask = reinterpret_cast<uint32_t*>(base_ptr);
-    int32_t* s_base = reinterpret_cast<int32_t*>(base_ptr + nb_cap * 64);
-    uint16_t* s_bits = reinterpret_cast<uint16_t*>(base_ptr + nb_cap * KMAP_STAGE_PER_LEAF);
+    // LDS slice of this wavefront: per staged leaf, 16 pairs (32-bit mask word, index of
+    // the word's first active voxel) -- 32-bit words keep the lookup's shifts and popcounts
+    // 32-bit and one 8-byte read fetches both -- then the row leaf's voxels in rank order
+    // as packed local coordinates x | y << 8 | z << 16
+    unsigned char* base_ptr = kmap_smem + (size_t)wave * (nb_cap * KMAP_STAGE_PER_LEAF + KMAP_ROW_BYTES);
+    uint2* s_mb = reinterpret_cast<uint2*>(base_ptr);
+    uint32_t* s_vox = reinterpret_cast<uint32_t*>(base_ptr + nb_cap * KMAP_STAGE_PER_LEAF);
     // per-tap offsets (x, y, z), shared by the workgroup
-    int32_t* s_off =
-        reinterpret_cast<int32_t*>(kmap_smem + (size_t)KMAP_WAVES * (nb_cap * KMAP_STAGE_PER_LEAF + 1024));
+    int32_t* s_off = reinterpret_cast<int32_t*>(kmap_smem +
+                                                (size_t)KMAP_WAVES * (nb_cap * KMAP_STAGE_PER_LEAF + KMAP_ROW_BYTES));
     for (int k = threadIdx.x; k < g.K; k += 64 * KMAP_WAVES) {
         const int tz = k % g.ext[2], ty = (k / g.ext[2]) % g.ext[1], tx = k / (g.ext[2] * g.ext[1]);
         s_off[3 * k + 0] = g.off0[0] + g.sgn * tx;
@@ -480,7 +481,9 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
         int rank = 0;
         for (int w = 0; w < 8; ++w) {
             const uint64_t m = rows.leaf_masks[L * 8 + w];
-            if ((m >> lane) & 1ull) s_bits[rank + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)(w * 64 + lane);
+            if ((m >> lane) & 1ull)
+                s_vox[rank + __popcll(m & ((1ull << lane) - 1ull))] =
+                    (uint32_t)w | ((uint32_t)(lane >> 3) << 8) | ((uint32_t)(lane & 7) << 16);
             rank += __popcll(m);
         }
         cnt = rank;
@@ -513,10 +516,8 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
 #pragma unroll
                 for (int w = 0; w < 8; ++w) {
                     const uint32_t lo32 = (uint32_t)mw[w], hi32 = (uint32_t)(mw[w] >> 32);
-                    s_mask[n * 16 + 2 * w] = lo32;
-                    s_mask[n * 16 + 2 * w + 1] = hi32;
-                    s_base[n * 16 + 2 * w] = bw[w];
-                    s_base[n * 16 + 2 * w + 1] = bw[w] + __popc(lo32);
+                    s_mb[n * 16 + 2 * w] = make_uint2(lo32, (uint32_t)bw[w]);
+                    s_mb[n * 16 + 2 * w + 1] = make_uint2(hi32, (uint32_t)(bw[w] + __popc(lo32)));
                 }
             }
         }
@@ -526,10 +527,14 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
 
     const int K = g.K;
     const int total = cnt * K;
-    int32_t* __restrict__ out = map + vbase * K;
-    // probe coordinate of row bit `bit` for tap offsets `of`; false when the division is inexact
-    auto probe_q = [&](int bit, const int (&of)[3], int32_t (&q)[3]) -> bool {
-        const int c[3] = {o[0] + (bit >> 6), o[1] + ((bit >> 3) & 7), o[2] + (bit & 7)};
+    // the leaf's first voxel is wavefront-uniform: keep the block's base address scalar
+    const int64_t vb = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(vbase >> 32)) << 32) |
+                                 (uint32_t)__builtin_amdgcn_readfirstlane((int)vbase));
+    int32_t* __restrict__ out = map + vb * K;
+    // probe coordinate of the row voxel with packed local coordinates `v` for tap offsets
+    // `of`; false when the division is inexact
+    auto probe_q = [&](uint32_t v, const int (&of)[3], int32_t (&q)[3]) -> bool {
+        const int c[3] = {o[0] + (int)(v & 255u), o[1] + (int)((v >> 8) & 255u), o[2] + (int)(v >> 16)};
         if constexpr (UNIT) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) q[a] = c[a] + of[a];
@@ -561,41 +566,72 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
         return n * 16 + (qb >> 5);
     };
     if (staged && K <= 64) {
-        // each lane keeps one tap (offsets in registers); a pass covers U groups of 64/K
-        // consecutive rows with every LDS read unconditional (clamped), so the reads of
-        // the U groups overlap
+        // each lane keeps one tap; a pass covers U groups of 64/K consecutive rows with
+        // every LDS read unconditional (row index clamped), so the reads of the U groups
+        // overlap
         constexpr int U = 4;
         const int rpi = 64 / K, lk = lane % K, lr = lane / K;
         const bool act = lane < rpi * K;
         const int of[3] = {s_off[3 * lk + 0], s_off[3 * lk + 1], s_off[3 * lk + 2]};
         const int last_r = cnt - 1;
+        if constexpr (UNIT) {
+            // Box coordinates P = local + (o - 8 lo + offset) per axis, packed one byte each
+            // (P < 8 * ext <= 255): one add per row, then the box leaf n and the bit inside
+            // it each come out of one 4 x u8 dot product.
+            const uint32_t add = (uint32_t)(o[0] - 8 * lo[0] + of[0]) | ((uint32_t)(o[1] - 8 * lo[1] + of[1]) << 8) |
+                                 ((uint32_t)(o[2] - 8 * lo[2] + of[2]) << 16);
+            const uint32_t leaf_w = (uint32_t)(ext[1] * ext[2]) | ((uint32_t)ext[2] << 8) | (1u << 16);
+            for (int r0 = lr; r0 < cnt; r0 += rpi * U) {
+                uint32_t P[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int rr = r0 + rpi * u;
+                    P[u] = s_vox[rr < last_r ? rr : last_r] + add;
+                }
+                uint2 mb[U];
+                int sh[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t n = __builtin_amdgcn_udot4((P[u] >> 3) & 0x1F1F1Fu, leaf_w, 0u, false);
+                    const uint32_t qb = __builtin_amdgcn_udot4(P[u] & 0x070707u, 0x00010840u, 0u, false);
+                    sh[u] = (int)(qb & 31u);
+                    mb[u] = s_mb[(n << 4) | (qb >> 5)];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int rr = r0 + rpi * u;
+                    const uint32_t m = mb[u].x;
+                    // index if the probe bit is set, else all ones (-1), without a branch
+                    const uint32_t idx = (mb[u].y + __popc(m & ((1u << sh[u]) - 1u))) | (((m >> sh[u]) & 1u) - 1u);
+                    if (act && rr < cnt) out[(uint32_t)(rr * K + lk)] = (int32_t)idx;
+                }
+            }
+            return;
+        }
         for (int r0 = lr; r0 < cnt; r0 += rpi * U) {
-            int bit[U];
+            uint32_t vox[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int rr = r0 + rpi * u;
-                bit[u] = s_bits[rr < last_r ? rr : last_r];
+                vox[u] = s_vox[rr < last_r ? rr : last_r];
             }
             int32_t q[U][3];
             int word[U], sh[U];
             bool ok[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                ok[u] = probe_q(bit[u], of, q[u]);
+                ok[u] = probe_q(vox[u], of, q[u]);
                 word[u] = ok[u] ? staged_word(q[u], sh[u]) : 0;
                 if (!ok[u]) sh[u] = 0;
             }
-            uint32_t m[U];
-            int32_t wb[U];
+            uint2 mb[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                m[u] = s_mask[word[u]];
-                wb[u] = s_base[word[u]];
-            }
+            for (int u = 0; u < U; ++u) mb[u] = s_mb[word[u]];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int rr = r0 + rpi * u;
-                if (act && rr < cnt) out[(int64_t)rr * K + lk] = staged_index(ok[u], q[u], m[u], wb[u], sh[u]);
+                if (act && rr < cnt)
+                    out[(int64_t)rr * K + lk] = staged_index(ok[u], q[u], mb[u].x, (int32_t)mb[u].y, sh[u]);
             }
         }
         return;
@@ -603,15 +639,16 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
     const int qstep = 64 / K, rstep = 64 % K;
     int r = lane / K, k = lane - (lane / K) * K;
     for (int idx = lane; idx < total; idx += 64) {
-        const int bit = s_bits[r];
+        const uint32_t vox = s_vox[r];
         const int of[3] = {s_off[3 * k + 0], s_off[3 * k + 1], s_off[3 * k + 2]};
         int32_t q[3];
-        const bool ok = probe_q(bit, of, q);
+        const bool ok = probe_q(vox, of, q);
         int32_t res = -1;
         if (staged) {
             int sh = 0;
             const int word = ok ? staged_word(q, sh) : 0;
-            res = staged_index(ok, q, s_mask[word], s_base[word], sh);
+            const uint2 mb = s_mb[word];
+            res = staged_index(ok, q, mb.x, (int32_t)mb.y, sh);
         } else if (ok && probe.n_leaves > 0) {
             res = voxel_index(probe, b, q[0], q[1], q[2]);
         }
@@ -1021,8 +1058,8 @@ static int kmap_launch(const fvdbk_grid* rows, const fvdbk_grid* probe, ProbeGeo
     }
     if (rows->n_leaves > 0) {
         int nb = kmap_box_bound(g);
-        if (nb * KMAP_STAGE_PER_LEAF + 1024 > KMAP_STAGE_BYTES) nb = 0;  // too large to stage: probe the hash per pair
-        const size_t lds = (size_t)KMAP_WAVES * (nb * KMAP_STAGE_PER_LEAF + 1024) + (size_t)K * 12;
+        if (nb * KMAP_STAGE_PER_LEAF + KMAP_ROW_BYTES > KMAP_STAGE_BYTES) nb = 0;  // too large to stage: probe the hash per pair
+        const size_t lds = (size_t)KMAP_WAVES * (nb * KMAP_STAGE_PER_LEAF + KMAP_ROW_BYTES) + (size_t)K * 12;
         FVDBK_REQUIRE(lds <= 65536, FVDBK_EINVAL, "kernel volume %lld too large for the kernel-map launch",
                       (long long)K);
         g.n_blocks = (int)((rows->n_leaves + KMAP_WAVES - 1) / KMAP_WAVES);
