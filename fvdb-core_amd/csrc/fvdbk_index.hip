@@ -417,13 +417,18 @@ struct ProbeGeom {
     int off0[3];
     int sgn;
     int n_blocks; // workgroups holding a row leaf (the launch may be padded to a multiple of 8)
+    int tail_len; // map entries past the last row (padding rows) that the launch's last
+                  // workgroup sets to -1 (it is always a padding workgroup), or 0
+    int64_t tail_off;
 };
 
 // Workgroup -> work-item order that gives each of the 8 XCDs (which take workgroups
-// round-robin) one contiguous range of the work, so spatially adjacent tiles share an
-// L2.  Returns -1 for padding workgroups.
+// round-robin) one contiguous range of the launch, so spatially adjacent tiles share an
+// L2.  The grid is a multiple of 8 workgroups; the map is a bijection of [0, grid), and
+// items >= n (returned as -1) are padding -- the last workgroup always is one when the
+// grid holds more than n workgroups.
 __device__ inline int64_t xcd_swizzle(int64_t bid, int64_t n) {
-    const int64_t per = (n + 7) / 8;
+    const int64_t per = (int64_t)gridDim.x / 8;
     const int64_t t = (bid % 8) * per + bid / 8;
     return t < n ? t : -1;
 }
@@ -450,6 +455,10 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
     extern __shared__ __attribute__((aligned(16))) unsigned char kmap_smem[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t blk = xcd_swizzle(blockIdx.x, g.n_blocks);
+    if (blk < 0 && g.tail_len > 0 && blockIdx.x == gridDim.x - 1) {
+        for (int i = threadIdx.x; i < g.tail_len; i += 64 * KMAP_WAVES) map[g.tail_off + i] = -1;
+        return;
+    }
     const int64_t L = blk < 0 ? rows.n_leaves : blk * KMAP_WAVES + wave;
     // LDS slice of this wavefront: per staged leaf, 16 pairs (32-bit mask word, index of
     // the word's first active voxel) -- 32-bit words keep the lookup's shifts and popcounts
@@ -1063,7 +1072,12 @@ static int kmap_launch(const fvdbk_grid* rows, const fvdbk_grid* probe, ProbeGeo
         FVDBK_REQUIRE(lds <= 65536, FVDBK_EINVAL, "kernel volume %lld too large for the kernel-map launch",
                       (long long)K);
         g.n_blocks = (int)((rows->n_leaves + KMAP_WAVES - 1) / KMAP_WAVES);
-        const unsigned blocks = (unsigned)((g.n_blocks + 7) / 8 * 8);
+        // a short padding tail is written by an extra (last) workgroup of the launch
+        const int64_t tail = (n_rows_padded - rows->n_voxels) * K;
+        const bool fold_tail = tail > 0 && tail <= 8192;
+        g.tail_len = fold_tail ? (int)tail : 0;
+        g.tail_off = rows->n_voxels * K;
+        const unsigned blocks = (unsigned)((g.n_blocks + (fold_tail ? 1 : 0) + 7) / 8 * 8);
         const bool unit = g.mul[0] == 1 && g.mul[1] == 1 && g.mul[2] == 1 && g.div[0] == 1 && g.div[1] == 1 &&
                           g.div[2] == 1;
         if (unit)
@@ -1073,6 +1087,7 @@ static int kmap_launch(const fvdbk_grid* rows, const fvdbk_grid* probe, ProbeGeo
             hipLaunchKernelGGL(k_kmap<false>, dim3(blocks), dim3(64 * KMAP_WAVES), lds, stream, *rows, *probe, g, nb,
                                map);
         FVDBK_LAUNCH_CHECK("k_kmap");
+        if (fold_tail) return FVDBK_OK;
     }
     const int64_t tail = (n_rows_padded - rows->n_voxels) * K;
     if (tail > 0) {
