@@ -302,7 +302,7 @@ def main() -> int:
             tr = json.load(f)
         roof["traffic"] = tr["hbm_bytes_per_launch"]
         roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
-    roof["kernel"] = ("k_conv_tile_dma<bf16, LX=3, NW=8> (conv A, 64->64, 3x3x3: 256-row workgroups, weights staged "
+    roof["kernel"] = ("k_conv_tile_dma<bf16, LX=3, NW=8, KT=27> (conv A, 64->64, 3x3x3: 256-row workgroups, weights staged "
                       "global->LDS, gathers 3 taps ahead, 4 waves/SIMD)")
     roof["avg_launch_ms"] = k_ms
     roof["algorithmic_flops"] = flops_a
