@@ -396,6 +396,53 @@ class GridBatch:
             idx = torch.where(idx >= 0, idx - base, idx)
         return j.jagged_like(idx)
 
+    def ijk_to_inv_index(self, ijk, cumulative: bool = False) -> JaggedTensor:
+        """For every voxel of this grid, the position of a query in ``ijk`` that names it
+        (within its own list, or in the whole batch with ``cumulative``), else -1
+        (``grid_batch.py:732-746``, ``IjkToInvIndex.cu``: ``out[voxel] = query``).  Where
+        several queries name one voxel the reference keeps whichever write lands last;
+        this keeps the largest position, so the result does not depend on timing."""
+        j = _as_jagged(ijk)
+        if j.num_tensors != self.grid_count:
+            raise ValueError("ijk must have one list per grid")
+        if j.jdata.dim() != 2 or j.jdata.shape[1] != 3:
+            raise ValueError("Expected 3 dimensional ijk (shape (n, 3))")
+        if j.jdata.shape[0] == 0:
+            raise ValueError("Empty tensor (ijk)")
+        if j.jdata.is_floating_point():
+            raise TypeError("ijk must have an integer type")
+        jid = _full_jidx(j).to(self.device)
+        idx = ext.ijk_to_index(*self._impl.tensors(), self.grid_count, j.jdata.to(self.device, torch.int32).contiguous(),
+                               jid).long()
+        pos = torch.arange(j.jdata.shape[0], dtype=torch.int64, device=self.device)
+        if not cumulative:
+            pos = pos - j.joffsets.to(self.device)[jid.long()]
+        out = torch.full((self.total_voxels,), -1, dtype=torch.int64, device=self.device)
+        hit = idx >= 0
+        out.scatter_reduce_(0, idx[hit], pos[hit], reduce="amax")
+        return self.jagged_like(out)
+
+    def inject_from_ijk(self, src_ijk, src, dst=None, default_value=0) -> JaggedTensor:
+        """Copy ``src`` (one row per coordinate of ``src_ijk``) onto the voxels of this grid
+        at those coordinates (``grid_batch.py:791-836``); coordinates outside the grid are
+        skipped.  ``dst`` is updated in place, or created filled with ``default_value``."""
+        if not isinstance(src_ijk, JaggedTensor):
+            raise TypeError(f"src_ijk must be a JaggedTensor, but got {type(src_ijk)}")
+        if not isinstance(src, JaggedTensor):
+            raise TypeError(f"src must be a JaggedTensor, but got {type(src)}")
+        if dst is None:
+            dst = self.jagged_like(torch.full((self.total_voxels,) + tuple(src.eshape), default_value, dtype=src.dtype,
+                                              device=src.device))
+        elif not isinstance(dst, JaggedTensor):
+            raise TypeError(f"dst must be a JaggedTensor, but got {type(dst)}")
+        if tuple(dst.eshape) != tuple(src.eshape):
+            raise ValueError(f"src and dst must have the same element shape, but got src: {tuple(src.eshape)}, "
+                             f"dst: {tuple(dst.eshape)}")
+        idx = self.ijk_to_index(src_ijk, cumulative=True).jdata
+        mask = idx >= 0
+        dst.jdata[idx[mask]] = src.jdata[mask]
+        return dst
+
     def coords_in_active_voxel(self, ijk) -> JaggedTensor:
         idx = self.ijk_to_index(ijk, cumulative=True)
         return idx.jagged_like(idx.jdata >= 0)

@@ -194,3 +194,49 @@ def test_coarsen_from_leaves_bit_exact(factor):
     t = small.conv_grid(2, 2)
     want = small.coarsened_grid(2)
     assert torch.equal(t.ijk.jdata, want.ijk.jdata) and torch.equal(t._impl.masks, want._impl.masks)
+
+
+def test_ijk_to_inv_index_and_inject_from_ijk():
+    """ijk_to_inv_index: per voxel, the position of the query naming it (per list or
+    cumulative), -1 where none does, the largest position among duplicate queries;
+    inject_from_ijk: rows of src land on the voxels at src_ijk, the rest keep dst or the
+    default.  Checked against dictionaries built on the host."""
+    fvdb = _fvdb()
+    rng = np.random.default_rng(12)
+    g, per = grids(fvdb, rng, [500, 300], 24)
+    offs = g.joffsets.cpu().tolist()
+    q_np = []
+    for b, p in enumerate(per):
+        pick = p[rng.integers(0, len(p), size=200)]                      # hits, with duplicates
+        miss = rng.integers(100, 120, size=(40, 3)).astype(np.int32)     # outside the grid
+        q = np.concatenate([pick, miss])[rng.permutation(240)]
+        q_np.append(q)
+    q = fvdb.JaggedTensor([torch.from_numpy(x).cuda() for x in q_np])
+    for cumulative in (False, True):
+        got = g.ijk_to_inv_index(q, cumulative=cumulative).jdata.cpu().numpy()
+        want = np.full(g.total_voxels, -1, dtype=np.int64)
+        base = 0
+        for b, (p, qq) in enumerate(zip(per, q_np)):
+            where = {tuple(c): offs[b] + i for i, c in enumerate(p.tolist())}
+            for e, c in enumerate(qq.tolist()):
+                v = where.get(tuple(c))
+                if v is not None:
+                    want[v] = max(want[v], e + (base if cumulative else 0))
+            base += len(qq)
+        assert np.array_equal(got, want), cumulative
+    # unique coordinates per list so that the expected result is unambiguous
+    uq = [np.unique(x, axis=0) for x in q_np]
+    sq = fvdb.JaggedTensor([torch.from_numpy(x).cuda() for x in uq])
+    sv = fvdb.JaggedTensor([torch.from_numpy(rng.standard_normal((len(x), 2))).cuda() for x in uq])
+    out = g.inject_from_ijk(sq, sv, default_value=-5.0).jdata.cpu().numpy()
+    want = np.full((g.total_voxels, 2), -5.0)
+    vals = [t.cpu().numpy() for t in [sv.jdata[sv.joffsets[i]:sv.joffsets[i + 1]] for i in range(len(uq))]]
+    for b, (p, x, v) in enumerate(zip(per, uq, vals)):
+        where = {tuple(c): offs[b] + i for i, c in enumerate(p.tolist())}
+        for c, row in zip(x.tolist(), v):
+            if tuple(c) in where:
+                want[where[tuple(c)]] = row
+    assert np.array_equal(out, want)
+    dst = g.jagged_like(torch.zeros(g.total_voxels, 2, dtype=torch.float64, device="cuda"))
+    res = g.inject_from_ijk(sq, sv, dst=dst)
+    assert res is dst and np.array_equal(dst.jdata.cpu().numpy(), np.where(want == -5.0, 0.0, want))
