@@ -463,12 +463,18 @@ __device__ inline void wait_vm_lds(int n) {
 //    buffers and the map tile) and is stored as whole rows.
 // With 8 wavefronts each weight tile is staged once for 256 rows, halving the staging
 // loads per row (they were a third of the kernel's vector memory instructions).
+// MI = 8: 128 output channels per workgroup (wavefronts of 32 rows x 128 channels), so
+// each gathered row feeds twice the MFMAs and layers of 128k output channels gather
+// every row half as often; 16 KB weight tiles, gathers LX = 2 taps ahead, and the
+// output leaves in two 64-channel halves through slices of the ring.
 // Features must span fewer than 2^31 bytes (32-bit buffer offsets); callers check.
-// Weight ring of k_conv_tile_dma: three separate LDS objects of 8 KB (file scope: a
-// static __shared__ array inside a kernel template does not link).
+// Weight rings of k_conv_tile_dma: three separate LDS objects of 8 KB (MI = 4) or 16 KB
+// (MI = 8) (file scope: a static __shared__ array inside a kernel template does not link;
+// a kernel is allocated only the objects it uses).
 static __shared__ __attribute__((aligned(16))) uint4 dma_ring0[512], dma_ring1[512], dma_ring2[512];
+static __shared__ __attribute__((aligned(16))) uint4 dma8_ring0[1024], dma8_ring1[1024], dma8_ring2[1024];
 
-template <typename T, int LX, int NW, int KT>
+template <typename T, int LX, int NW, int KT, int MI = 4>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict__ A, const int32_t* __restrict__ map,
                      int64_t n_rows, int M, int n_mt, int64_t n_tiles, T* __restrict__ Y, const T* __restrict__ bias,
@@ -476,20 +482,22 @@ void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict_
     static_assert(sizeof(T) == 2, "16-bit features");
     using F = Frag<T>;
     using V = typename F::vec;
-    constexpr int MI = 4, NI = 2, CIN = 64;
+    constexpr int NI = 2, CIN = 64;
     constexpr int RT = NW * NI * 16;            // rows per workgroup
-    constexpr int WT = MI * 16 * CIN;          // elements of one tap tile (8 KB)
+    constexpr int WT = MI * 16 * CIN;          // elements of one tap tile (8 or 16 KB)
     constexpr int NCH = CIN / F::CH;
     constexpr int GPT = NI * NCH;              // gather loads per tap and lane
-    constexpr int OUT_SLICE = NI * 16 * (MI * 16 + 8);
+    constexpr int OUT_SLICE = NI * 16 * (4 * 16 + 8);  // one 64-channel output slice
     static_assert(NW == 4 || NW == 8, "4 or 8 wavefronts");
+    static_assert(MI == 4 || MI == 8, "64 or 128 output channels");
     static_assert(KT > LX, "more taps than the gather lookahead");
     static_assert(OUT_SLICE <= WT, "output slices fit the weight buffers");
-    static_assert(WT * sizeof(T) == sizeof(dma_ring0), "ring buffer size");
+    static_assert(MI == 4 || 3 * OUT_SLICE <= WT, "three slices per 16 KB buffer");
+    static_assert(WT * sizeof(T) == (MI == 4 ? sizeof(dma_ring0) : sizeof(dma8_ring0)), "ring buffer size");
     extern __shared__ __attribute__((aligned(16))) unsigned char tile_smem[];
-    T* const sW0 = reinterpret_cast<T*>(dma_ring0);
-    T* const sW1 = reinterpret_cast<T*>(dma_ring1);
-    T* const sW2 = reinterpret_cast<T*>(dma_ring2);
+    T* const sW0 = reinterpret_cast<T*>(MI == 4 ? dma_ring0 : dma8_ring0);
+    T* const sW1 = reinterpret_cast<T*>(MI == 4 ? dma_ring1 : dma8_ring1);
+    T* const sW2 = reinterpret_cast<T*>(MI == 4 ? dma_ring2 : dma8_ring2);
     int32_t* sMap = reinterpret_cast<int32_t*>(tile_smem);
     auto wbuf = [&](int k) -> T* {
         const int r = k % 3;
@@ -514,9 +522,9 @@ void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict_
 #pragma unroll
         for (int j = 0; j < NI; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    // staging: the 64 weight rows are 8 blocks of 8; wavefront w copies blocks w*SQ .. +SQ-1,
-    // lane -> (row, position)
-    constexpr int SQ = 8 / NW;                 // staging loads per lane per tap
+    // staging: the MI*16 weight rows are 2*MI blocks of 8; wavefront w copies blocks
+    // w*SQ .. +SQ-1, lane -> (row, position)
+    constexpr int SQ = 2 * MI / NW;            // staging loads per lane per tap
     const T* wsrc[SQ];
 #pragma unroll
     for (int q = 0; q < SQ; ++q) {
@@ -583,19 +591,32 @@ void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict_
         }
     }
     __syncthreads();  // every wavefront is done with the weight buffers and the map tile
-    T* slice = wave == 0 ? sW0
-               : wave == 1 ? sW1
-               : wave == 2 ? sW2
-                           : reinterpret_cast<T*>(tile_smem) + (wave - 3) * OUT_SLICE;
-    store_tile_slice<T, MI, NI>(Y, acc, slice, lane, row0 + wave * NI * 16, nullptr, n_rows, M, m0, bias, relu);
+    if constexpr (MI == 4) {
+        T* slice = wave == 0 ? sW0
+                   : wave == 1 ? sW1
+                   : wave == 2 ? sW2
+                               : reinterpret_cast<T*>(tile_smem) + (wave - 3) * OUT_SLICE;
+        store_tile_slice<T, MI, NI>(Y, acc, slice, lane, row0 + wave * NI * 16, nullptr, n_rows, M, m0, bias, relu);
+    } else {
+        // three slices per ring buffer; the wavefront stores its two 64-channel halves in turn
+        T* const ring = wave < 3 ? sW0 : (wave < 6 ? sW1 : sW2);
+        T* slice = ring + (wave % 3) * OUT_SLICE;
+        using Half = floatx4[4][NI];
+        store_tile_slice<T, 4, NI>(Y, *reinterpret_cast<const Half*>(&acc[0][0]), slice, lane, row0 + wave * NI * 16,
+                                   nullptr, n_rows, M, m0, bias, relu);
+        store_tile_slice<T, 4, NI>(Y, *reinterpret_cast<const Half*>(&acc[4][0]), slice, lane, row0 + wave * NI * 16,
+                                   nullptr, n_rows, M, m0 + 64, bias, relu);
+    }
 }
 
 // Launches k_conv_tile_dma when the layer fits it (K = 27 or 8); false otherwise.  Large
 // layers take 8-wavefront workgroups of 256 rows, which stage each weight tile once for
 // twice the rows (conv A of the 2.3M-voxel sphere batch: -7..-12% on MI355X); smaller
-// ones keep 128-row workgroups so that the launch still fills the chip.  Dynamic LDS
-// holds the map tile, and at the end the output slices of wavefronts 3.. (wavefronts
-// 0-2 use the weight ring).
+// ones keep 128-row workgroups so that the launch still fills the chip.  Large 2x2x2
+// layers with 128k output channels take 128-channel workgroups (conv B of the sphere
+// batch, 64->128: 0.155 -> 0.129 ms on MI355X).  Dynamic LDS holds the map tile, and at
+// the end the output slices of wavefronts 3.. (wavefronts 0-2 use the weight ring; with
+// 128-channel workgroups every slice is in the ring).
 template <typename T>
 static bool launch_tile_dma(const T* X, int64_t n_in, const T* A, const int32_t* map, int64_t n_rows, int K, int M,
                             T* Y, const T* bias, int relu, hipStream_t s) {
@@ -616,6 +637,16 @@ static bool launch_tile_dma(const T* X, int64_t n_in, const T* A, const int32_t*
     using W4 = std::integral_constant<int, 4>;
     using W8 = std::integral_constant<int, 8>;
     const bool wide = (n_rows + 255) / 256 * n_mt >= 1024;
+    if (K == 8 && M % 128 == 0 && (n_rows + 255) / 256 * (M / 128) >= 1024) {
+        constexpr int NW = 8, KT = 8;
+        const int n_mt8 = M / 128;
+        const int64_t n_tiles = (n_rows + 32 * NW - 1) / (32 * NW) * n_mt8;
+        const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
+        const size_t lds = (size_t)32 * NW * KT * 4;  // map tile (output slices live in the ring)
+        hipLaunchKernelGGL((k_conv_tile_dma<T, 2, NW, KT, 8>), grid, dim3(64 * NW), lds, s, X, n_in, A, map, n_rows,
+                           M, n_mt8, n_tiles, Y, bias, relu);
+        return true;
+    }
     if (K == 27) {
         if (wide) go(W8{}, std::integral_constant<int, 27>{});
         else go(W4{}, std::integral_constant<int, 27>{});

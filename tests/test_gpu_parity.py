@@ -401,14 +401,15 @@ def test_sorted_rows_bit_exact(dtype, cin, cout):
 
 
 @pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
-@pytest.mark.parametrize("cout,large", [(64, False), (192, False), (192, True)])
+@pytest.mark.parametrize("cout,large", [(64, False), (192, False), (192, True), (256, True)])
 @pytest.mark.parametrize("ks,st", [(3, 1), (2, 2)])
 def test_direct_lds_tile_bit_exact(dtype, cout, large, ks, st):
     """The direct-to-LDS contraction (16-bit, 64 input channels, 27 or 8 taps, rows in tree
     order, outputs in whole 64-channel tiles) equals the tap-skipping kernel run over an
     identity row order bit for bit (same accumulation order; a skipped tap adds exact zeros),
     with and without the fused bias + ReLU, on a ragged last tile, and the fp64 oracle within
-    the 16-bit bound.  The large grid (>= 1024 tiles of 256 rows) takes the 8-wavefront form."""
+    the 16-bit bound.  The large grid (>= 1024 tiles of 256 rows) takes the 8-wavefront form;
+    2x2x2 layers with 128k outputs there take 128-channel workgroups."""
     fvdb = _fvdb()
     from fvdb._lib import ext
     from fvdb.utils.synthetic import sphere_points
@@ -431,6 +432,8 @@ def test_direct_lds_tile_bit_exact(dtype, cout, large, ks, st):
         assert torch.equal(y, y_skip), (b is not None, relu)
     if large:
         assert (n + 255) // 256 * (cout // 64) >= 1024, n
+        if ks == 2 and cout % 128 == 0:
+            assert (n + 255) // 256 * (cout // 128) >= 1024, n
         return
     ref = oracle.sparse_conv(x.double().cpu(), w.double().cpu(), kmap[:n].cpu().numpy(), n, st)
     assert rel_err(ext.implicit_gemm(x, a, kmap, n), ref) < REL_TOL[dtype]
