@@ -459,7 +459,8 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
         for (int i = threadIdx.x; i < g.tail_len; i += 64 * KMAP_WAVES) map[g.tail_off + i] = -1;
         return;
     }
-    const int64_t L = blk < 0 ? rows.n_leaves : blk * KMAP_WAVES + wave;
+    // the row leaf is wavefront-uniform: its key, masks and box live in scalar registers
+    const int64_t L = blk < 0 ? rows.n_leaves : blk * KMAP_WAVES + __builtin_amdgcn_readfirstlane(wave);
     // LDS slice of this wavefront: per staged leaf, 16 pairs (32-bit mask word, index of
     // the word's first active voxel) -- 32-bit words keep the lookup's shifts and popcounts
     // 32-bit and one 8-byte read fetches both -- then the row leaf's voxels in rank order
@@ -470,8 +471,15 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
     // per-tap offsets (x, y, z), shared by the workgroup
     int32_t* s_off = reinterpret_cast<int32_t*>(kmap_smem +
                                                 (size_t)KMAP_WAVES * (nb_cap * KMAP_STAGE_PER_LEAF + KMAP_ROW_BYTES));
+    // tap k -> (tx, ty, tz), z fastest; the quotients by the small extents come from float
+    // reciprocals: k < 4096, so (k + 0.5) / e is at least 0.5 / e from an integer, far more
+    // than the rounding error of the product, and the truncation is exact
+    const float r2 = 1.0f / (float)g.ext[2], r12 = 1.0f / (float)(g.ext[1] * g.ext[2]);
     for (int k = threadIdx.x; k < g.K; k += 64 * KMAP_WAVES) {
-        const int tz = k % g.ext[2], ty = (k / g.ext[2]) % g.ext[1], tx = k / (g.ext[2] * g.ext[1]);
+        const int tx = (int)(((float)k + 0.5f) * r12);
+        const int rem = k - tx * g.ext[1] * g.ext[2];
+        const int ty = (int)(((float)rem + 0.5f) * r2);
+        const int tz = rem - ty * g.ext[2];
         s_off[3 * k + 0] = g.off0[0] + g.sgn * tx;
         s_off[3 * k + 1] = g.off0[1] + g.sgn * ty;
         s_off[3 * k + 2] = g.off0[2] + g.sgn * tz;
@@ -500,16 +508,21 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
             const int span = g.sgn * (g.ext[a] - 1);
             const int off_lo = g.off0[a] + (span < 0 ? span : 0);
             const int off_hi = g.off0[a] + (span > 0 ? span : 0);
-            const int q_lo = floordiv(o[a] * g.mul[a] + off_lo, g.div[a]);
-            const int q_hi = floordiv((o[a] + 7) * g.mul[a] + off_hi, g.div[a]);
+            const int q_lo = UNIT ? o[a] + off_lo : floordiv(o[a] * g.mul[a] + off_lo, g.div[a]);
+            const int q_hi = UNIT ? o[a] + 7 + off_hi : floordiv((o[a] + 7) * g.mul[a] + off_hi, g.div[a]);
             lo[a] = q_lo >> 3;
             ext[a] = (q_hi >> 3) - lo[a] + 1;
         }
         const int nn = ext[0] * ext[1] * ext[2];
         staged = nn <= nb_cap;
         if (staged) {
+            // box leaf n -> (dx, dy, dz), z fastest (n < nb_cap: exact float quotients as above)
+            const float e2r = 1.0f / (float)ext[2], e12r = 1.0f / (float)(ext[1] * ext[2]);
             for (int n = lane; n < nn; n += 64) {
-                const int dz = n % ext[2], dy = (n / ext[2]) % ext[1], dx = n / (ext[2] * ext[1]);
+                const int dx = (int)(((float)n + 0.5f) * e12r);
+                const int nrem = n - dx * ext[1] * ext[2];
+                const int dy = (int)(((float)nrem + 0.5f) * e2r);
+                const int dz = nrem - dy * ext[2];
                 const int32_t lx = (lo[0] + dx) * 8, ly = (lo[1] + dy) * 8, lz = (lo[2] + dz) * 8;
                 int32_t leaf = -1;
                 if (probe.n_leaves > 0 && lx >= kCoordMin && lx <= kCoordMax && ly >= kCoordMin && ly <= kCoordMax &&
