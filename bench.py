@@ -453,7 +453,7 @@ def run_diff(torch, fvdb, dev, args):
         kio = weight_to_kio(w, False)
         a_f = kio.transpose(1, 2).contiguous()
         # row orders as the autograd path picks them (tap-mask order per window, or tree order)
-        of, ob = pack._order_for(True, 128, dt), pack._order_for(False, 128, dt)
+        of, ob = pack._order_for(True, 128, dt, 128), pack._order_for(False, 128, dt, 128)
         fwd = lambda: ext.implicit_gemm(x, a_f, pack._map_fwd(), N, of)  # noqa: E731
         dgr = lambda: ext.implicit_gemm(R, kio, pack._map_bwd(), N, ob)  # noqa: E731
         chunk = wgrad_chunk(P, dt)
@@ -461,7 +461,7 @@ def run_diff(torch, fvdb, dev, args):
         t = {"fwd_ms": _median_ms(torch, fwd, 10), "dgrad_ms": _median_ms(torch, dgr, 10),
              "wgrad_ms": _median_ms(torch, wgr, 10)}
         t["tflops"] = {k[:-3]: F / (v * 1e-3) / 1e12 for k, v in list(t.items())}
-        t["row_order"] = "tap mask per 8192-row window" if of is not None else "tree"
+        t["row_order"] = "tap mask per 8192-row window" if of is not None else "tree (direct-to-LDS kernel)"
         # autograd end to end: loss = sum(out * R)
         xr = x.clone().requires_grad_(True)
         wr = w.clone().requires_grad_(True)
@@ -555,7 +555,7 @@ def run_orders(torch, fvdb, dev, args):
                 raise RuntimeError(f"{name} {dn}: row order changed the result")
             rec[dn] = {"tree_ms": _median_ms(torch, lambda: ext.implicit_gemm(x, a, m, n_rows), 10),
                        "windowed_ms": _median_ms(torch, lambda: ext.implicit_gemm(x, a, m, n_rows, order), 10),
-                       "picked": "windowed" if wants_row_order(K, cin, dt) else "tree"}
+                       "picked": "windowed" if wants_row_order(K, cin, dt, cout, n_rows) else "tree"}
         log(f"[bench] orders {rec}")
         rows.append(rec)
 

@@ -467,6 +467,9 @@ __device__ inline void wait_vm_lds(int n) {
 // each gathered row feeds twice the MFMAs and layers of 128k output channels gather
 // every row half as often; 16 KB weight tiles, gathers LX = 2 taps ahead, and the
 // output leaves in two 64-channel halves through slices of the ring.
+// NCC = 2 (128 input channels, with MI = 8): each pipeline step is one (tap, 64-channel
+// chunk) pair, chunk fastest -- the accumulation order of k_conv_tile_c, so results are
+// bit-identical to it -- with one step of gather lookahead (54 steps for 3x3x3).
 // Features must span fewer than 2^31 bytes (32-bit buffer offsets); callers check.
 // Weight rings of k_conv_tile_dma: three separate LDS objects of 8 KB (MI = 4) or 16 KB
 // (MI = 8) (file scope: a static __shared__ array inside a kernel template does not link;
@@ -474,7 +477,7 @@ __device__ inline void wait_vm_lds(int n) {
 static __shared__ __attribute__((aligned(16))) uint4 dma_ring0[512], dma_ring1[512], dma_ring2[512];
 static __shared__ __attribute__((aligned(16))) uint4 dma8_ring0[1024], dma8_ring1[1024], dma8_ring2[1024];
 
-template <typename T, int LX, int NW, int KT, int MI = 4>
+template <typename T, int LX, int NW, int KT, int MI = 4, int NCC = 1>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict__ A, const int32_t* __restrict__ map,
                      int64_t n_rows, int M, int n_mt, int64_t n_tiles, T* __restrict__ Y, const T* __restrict__ bias,
@@ -482,15 +485,17 @@ void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict_
     static_assert(sizeof(T) == 2, "16-bit features");
     using F = Frag<T>;
     using V = typename F::vec;
-    constexpr int NI = 2, CIN = 64;
+    // CIN = NCC chunks of CC = 64 channels; a pipeline step v is (tap v / NCC, chunk v % NCC)
+    constexpr int NI = 2, CC = 64, CIN = CC * NCC, KV = KT * NCC;
     constexpr int RT = NW * NI * 16;            // rows per workgroup
-    constexpr int WT = MI * 16 * CIN;          // elements of one tap tile (8 or 16 KB)
-    constexpr int NCH = CIN / F::CH;
+    constexpr int WT = MI * 16 * CC;           // elements of one step's weight tile (8 or 16 KB)
+    constexpr int NCH = CC / F::CH;
     constexpr int GPT = NI * NCH;              // gather loads per tap and lane
     constexpr int OUT_SLICE = NI * 16 * (4 * 16 + 8);  // one 64-channel output slice
     static_assert(NW == 4 || NW == 8, "4 or 8 wavefronts");
     static_assert(MI == 4 || MI == 8, "64 or 128 output channels");
-    static_assert(KT > LX, "more taps than the gather lookahead");
+    static_assert(KV > LX, "more steps than the gather lookahead");
+    static_assert(NCC == 1 || MI == 8, "wide inputs take 128-channel workgroups");
     static_assert(OUT_SLICE <= WT, "output slices fit the weight buffers");
     static_assert(MI == 4 || 3 * OUT_SLICE <= WT, "three slices per 16 KB buffer");
     static_assert(WT * sizeof(T) == (MI == 4 ? sizeof(dma_ring0) : sizeof(dma8_ring0)), "ring buffer size");
@@ -499,8 +504,8 @@ void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict_
     T* const sW1 = reinterpret_cast<T*>(MI == 4 ? dma_ring1 : dma8_ring1);
     T* const sW2 = reinterpret_cast<T*>(MI == 4 ? dma_ring2 : dma8_ring2);
     int32_t* sMap = reinterpret_cast<int32_t*>(tile_smem);
-    auto wbuf = [&](int k) -> T* {
-        const int r = k % 3;
+    auto wbuf = [&](int v) -> T* {
+        const int r = v % 3;
         return r == 0 ? sW0 : (r == 1 ? sW1 : sW2);
     };
 
@@ -532,21 +537,23 @@ void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict_
         const int seg = (lane & 7) ^ (lane >> 3);
         wsrc[q] = A + (int64_t)(m0 + row) * CIN + seg * 8;
     }
-    auto stage = [&](int k) {
-        T* dst = wbuf(k) + wave * SQ * 512;
+    auto stage = [&](int v) {
+        const int k = v / NCC, c = v % NCC;
+        T* dst = wbuf(v) + wave * SQ * 512;
 #pragma unroll
         for (int q = 0; q < SQ; ++q)
-            lds_dma16(wsrc[q] + (int64_t)k * M * CIN, dst + q * 512);
+            lds_dma16(wsrc[q] + (int64_t)k * M * CIN + c * CC, dst + q * 512);
     };
     const int bytes = (int)(n_in * CIN * (int64_t)sizeof(T));
     const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, bytes, 0x00020000);
     constexpr int MISSING = 0x7FFFFFC0;        // beyond any buffer: the load returns zeros
     V b[LX + 1][NI][NCH];
-    auto gather = [&](int k, V (&bb)[NI][NCH]) {
+    auto gather = [&](int v, V (&bb)[NI][NCH]) {
+        const int k = v / NCC, c = v % NCC;
 #pragma unroll
         for (int j = 0; j < NI; ++j) {
             const int32_t src = sMap[(wave * NI * 16 + j * 16 + r16) * KT + k];
-            const int off = src >= 0 ? src * (CIN * (int)sizeof(T)) + g * 16 : MISSING;
+            const int off = src >= 0 ? src * (CIN * (int)sizeof(T)) + c * (CC * (int)sizeof(T)) + g * 16 : MISSING;
 #pragma unroll
             for (int ch = 0; ch < NCH; ++ch)
                 bb[j][ch] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(xr, off + ch * 64, 0, 0));
@@ -554,39 +561,39 @@ void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict_
     };
     // Loads issued after the staging of tap k+1, up to the end of tap k: issue order is
     // stage 0, stage 1, gathers 0..LX-1, then per tap t: stage t+2, gathers t+LX.
-    auto after_stage = [&](int k) -> int {
-        auto G = [&](int tt) { return tt + LX < KT ? GPT : 0; };
-        auto D = [&](int tt) { return tt + 2 < KT ? SQ : 0; };
-        return k == 0 ? GPT * LX + D(0) + G(0) : G(k - 1) + D(k) + G(k);
+    auto after_stage = [&](int v) -> int {
+        auto G = [&](int tt) { return tt + LX < KV ? GPT : 0; };
+        auto D = [&](int tt) { return tt + 2 < KV ? SQ : 0; };
+        return v == 0 ? GPT * LX + D(0) + G(0) : G(v - 1) + D(v) + G(v);
     };
 
     __syncthreads();  // map tile visible
     stage(0);
     stage(1);
 #pragma unroll
-    for (int k = 0; k < LX; ++k) gather(k, b[k]);
+    for (int v = 0; v < LX; ++v) gather(v, b[v]);
     wait_vm_lds<0>();
     wg_barrier();
 #pragma unroll
-    for (int k = 0; k < KT; ++k) {
-        if (k + 2 < KT) stage(k + 2);
-        if (k + LX < KT) gather(k + LX, b[(k + LX) % (LX + 1)]);
-        const T* wb = wbuf(k);
+    for (int v = 0; v < KV; ++v) {
+        if (v + 2 < KV) stage(v + 2);
+        if (v + LX < KV) gather(v + LX, b[(v + LX) % (LX + 1)]);
+        const T* wb = wbuf(v);
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) {
 #pragma unroll
             for (int i = 0; i < MI; ++i) {
                 const int pos = (ch * 4 + g) ^ (r16 & 7);
-                const V a = *reinterpret_cast<const V*>(wb + (i * 16 + r16) * CIN + pos * 8);
+                const V a = *reinterpret_cast<const V*>(wb + (i * 16 + r16) * CC + pos * 8);
 #pragma unroll
-                for (int j = 0; j < NI; ++j) mfma(a, b[k % (LX + 1)][j][ch], acc[i][j]);
+                for (int j = 0; j < NI; ++j) mfma(a, b[v % (LX + 1)][j][ch], acc[i][j]);
             }
         }
-        if (k + 1 < KT) {
-            // tap k+1's weights have landed in every wavefront; nobody reads tap k's buffer
-            // again before stage(k+3) overwrites it
+        if (v + 1 < KV) {
+            // step v+1's weights have landed in every wavefront; nobody reads step v's buffer
+            // again before stage(v+3) overwrites it
             __builtin_amdgcn_sched_barrier(0);
-            wait_vm_lds(after_stage(k));
+            wait_vm_lds(after_stage(v));
             wg_barrier();
         }
     }
@@ -654,6 +661,27 @@ static bool launch_tile_dma(const T* X, int64_t n_in, const T* A, const int32_t*
         if (wide) go(W8{}, std::integral_constant<int, 8>{});
         else go(W4{}, std::integral_constant<int, 8>{});
     }
+    return true;
+}
+
+// Direct-to-LDS contraction for 16-bit 3x3x3 layers with 128 input channels and 128k output
+// channels in tree order (NCC = 2, 128-channel workgroups of 8 wavefronts, one step of
+// lookahead); large layers only (>= 1024 such tiles), false otherwise.  Planes + spheres
+// 128 -> 128 (1.1M rows): 0.71 ms against 1.04 ms for the tap-skipping kernel over rows in
+// tap-mask order and 1.74 ms for the register-staged kernel in tree order (MI355X).
+template <typename T>
+static bool launch_tile_dma_wide(const T* X, int64_t n_in, const T* A, const int32_t* map, int64_t n_rows, int K,
+                                 int M, T* Y, const T* bias, int relu, hipStream_t s) {
+    if (K != 27 || M % 128 != 0 || n_in * 128 * (int64_t)sizeof(T) > 0x7FFFFF00 || ((uintptr_t)Y % 16) != 0 ||
+        (n_rows + 255) / 256 * (M / 128) < 1024)
+        return false;
+    constexpr int NW = 8, KT = 27;
+    const int n_mt = M / 128;
+    const int64_t n_tiles = (n_rows + 32 * NW - 1) / (32 * NW) * n_mt;
+    const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
+    const size_t lds = (size_t)32 * NW * KT * 4;  // map tile (output slices live in the ring)
+    hipLaunchKernelGGL((k_conv_tile_dma<T, 1, NW, KT, 8, 2>), grid, dim3(64 * NW), lds, s, X, n_in, A, map, n_rows, M,
+                       n_mt, n_tiles, Y, bias, relu);
     return true;
 }
 
@@ -1021,6 +1049,9 @@ inline bool conv_tile_dispatch_t(const T* X, int64_t n_in, int cin, const T* A, 
     case 32: return try_tile<T, 32>(X, n_in, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     case 64: return try_tile<T, 64>(X, n_in, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     case 128:
+        if constexpr (sizeof(T) == 2) {
+            if (perm == nullptr && launch_tile_dma_wide<T>(X, n_in, A, map, n_rows, K, M, Y, bias, relu, s)) return true;
+        }
         if (K == 27 || K == 8) return try_tile_c<T, 2>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
         return try_tile_looped<T, 128>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     case 256:

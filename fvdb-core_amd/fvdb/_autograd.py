@@ -84,12 +84,12 @@ class SparseConvFunction(torch.autograd.Function):
     def forward(ctx, feats, weight, bias, pack, transposed: bool, relu: bool = False):
         n_src, n_tgt = pack.source_grid.total_voxels, pack.target_grid.total_voxels
         cin, dt = feats.shape[1], feats.dtype
+        koi = weight_to_koi(weight, transposed)
         if not transposed:
-            fmap, order, n_out = pack._map_fwd(), pack._order_for(True, cin, dt), n_tgt
+            fmap, order, n_out = pack._map_fwd(), pack._order_for(True, cin, dt, koi.shape[1]), n_tgt
         else:
-            fmap, order, n_out = pack._map_bwd(), pack._order_for(False, cin, dt), n_src
-        out = ext.implicit_gemm(feats.contiguous(), weight_to_koi(weight, transposed), fmap, n_out, order, bias,
-                                bool(relu))
+            fmap, order, n_out = pack._map_bwd(), pack._order_for(False, cin, dt, koi.shape[1]), n_src
+        out = ext.implicit_gemm(feats.contiguous(), koi, fmap, n_out, order, bias, bool(relu))
         # the [K, I, O] operand of dgrad is built from the weight in backward, when needed
         ctx.save_for_backward(feats, weight, out if relu else None)
         ctx.pack = pack
@@ -115,10 +115,11 @@ class SparseConvFunction(torch.autograd.Function):
         gin = gw = None
         if ctx.needs_input_grad[0]:
             cg, dt = grad_out.shape[1], grad_out.dtype
+            co = kio.shape[1]  # the dgrad contraction's output channels (the layer's inputs)
             if not transposed:
-                dmap, order, n_in = pack._map_bwd(), pack._order_for(False, cg, dt), pack.source_grid.total_voxels
+                dmap, order, n_in = pack._map_bwd(), pack._order_for(False, cg, dt, co), pack.source_grid.total_voxels
             else:
-                dmap, order, n_in = pack._map_fwd(), pack._order_for(True, cg, dt), pack.target_grid.total_voxels
+                dmap, order, n_in = pack._map_fwd(), pack._order_for(True, cg, dt, co), pack.target_grid.total_voxels
             gin = ext.implicit_gemm(grad_out, kio, dmap, n_in, order)
         if ctx.needs_input_grad[1]:
             pairs, sizes = pack._pairs()

@@ -342,7 +342,7 @@ class UNetExecutor:
                 a, b = wb
                 # rows grouped by tap mask per window where that pays (same rule as the packs)
                 order = None
-                if n_rows > 0 and wants_row_order(m.shape[1], inp.shape[1], inp.dtype):
+                if n_rows > 0 and wants_row_order(m.shape[1], inp.shape[1], inp.dtype, a.shape[1], n_rows):
                     if id(m) not in orders:
                         orders[id(m)] = ext.row_order(m, n_rows, AUTO_ORDER_WINDOW)
                     order = orders[id(m)]

@@ -53,16 +53,24 @@ def _pad128(n: int) -> int:
 AUTO_ORDER_WINDOW = 8192
 
 
-def wants_row_order(kernel_volume: int, cin: int, dtype: torch.dtype) -> bool:
+def wants_row_order(kernel_volume: int, cin: int, dtype: torch.dtype, cout: int = 0, rows: int = 0) -> bool:
     """Whether a contraction runs faster with its rows ordered by tap mask per window
     (measured on MI355X): fp32 (MFMA-bound, so each skipped tap is saved time) and
     16-bit inputs of 128+ channels.  Narrower 16-bit layers stay in tree order, whose
     gathers hit cache; that includes maps with one tap per row (a 2x2x2 stride-2
-    kernel's fine-to-coarse map), where tree order computes 8 taps per row.  Only the
-    kernel volumes with tap-skipping kernels (27, 8) qualify; fp64 runs the scalar kernel."""
+    kernel's fine-to-coarse map), where tree order computes 8 taps per row.  Large 16-bit
+    3x3x3 layers of 128 input and 128k output channels (``cout``; at least 1024
+    workgroups of 256 ``rows``) also stay in tree order: the direct-to-LDS kernel runs them
+    faster there (planes + spheres 128 -> 128: 0.71 vs 1.04 ms).  Only the kernel volumes
+    with tap-skipping kernels (27, 8) qualify; fp64 runs the scalar kernel."""
     if kernel_volume not in (8, 27) or dtype == torch.float64:
         return False
-    return dtype == torch.float32 or cin >= 128
+    if dtype == torch.float32:
+        return True
+    if (kernel_volume == 27 and cin == 128 and cout > 0 and cout % 128 == 0
+            and (rows + 255) // 256 * (cout // 128) >= 1024):
+        return False
+    return cin >= 128
 
 
 class SparseConvPackInfo:
@@ -157,14 +165,15 @@ class SparseConvPackInfo:
     # of AUTO_ORDER_WINDOW consecutive rows (neighbours stay close in memory) where
     # ``wants_row_order`` says so.  Narrow 16-bit layers stay in tree order, whose
     # gathers hit cache.
-    def _order_for(self, fwd: bool, cin: int, dtype: torch.dtype) -> Optional[torch.Tensor]:
+    def _order_for(self, fwd: bool, cin: int, dtype: torch.dtype, cout: int = 0) -> Optional[torch.Tensor]:
         """Row order for a contraction over the forward (rows = target voxels) or
-        transposed (rows = source voxels) map with ``cin`` input channels, or None."""
+        transposed (rows = source voxels) map with ``cin`` input and ``cout`` output
+        channels, or None."""
         if self._sorted:
             return self._order_fwd() if fwd else self._order_bwd()
-        if not wants_row_order(self.kernel_volume, cin, dtype):
-            return None
         n = self._tgt.total_voxels if fwd else self._src.total_voxels
+        if not wants_row_order(self.kernel_volume, cin, dtype, cout, n):
+            return None
         if n == 0:
             return None
         if self._ord_auto.get(fwd) is None:

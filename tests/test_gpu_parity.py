@@ -439,6 +439,36 @@ def test_direct_lds_tile_bit_exact(dtype, cout, large, ks, st):
     assert rel_err(ext.implicit_gemm(x, a, kmap, n), ref) < REL_TOL[dtype]
 
 
+@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
+def test_direct_lds_wide_bit_exact(dtype):
+    """The direct-to-LDS contraction for 128 input channels (steps of one tap and one
+    64-channel chunk, 128-channel workgroups) equals the tap-skipping kernel over an
+    identity row order bit for bit, with and without the fused bias + ReLU, on a layer
+    large enough to take it (>= 1024 workgroups of 256 rows), forward and as dgrad
+    (transposed map)."""
+    fvdb = _fvdb()
+    from fvdb._lib import ext
+    from fvdb.utils.synthetic import sphere_points
+
+    grid = fvdb.GridBatch.from_points(sphere_points(seed=5, n_points=1_500_000, device="cuda"), voxel_sizes=1 / 512)
+    pack, _ = grid.sparse_conv_kernel_map(3, 1)
+    pack.build_implicit_gemm(training=True)
+    n, cout = grid.total_voxels, 256
+    assert (n + 255) // 256 * (cout // 128) >= 1024, n
+    print(f"[direct-lds-wide] {dtype} rows={n} (last tile {n % 256} rows)", flush=True)
+    g = torch.Generator().manual_seed(22)
+    x = torch.randn(n, 128, generator=g).to("cuda", dtype)
+    a = (0.05 * torch.randn(27, cout, 128, generator=g)).to("cuda", dtype)
+    bias = torch.randn(cout, generator=g).to("cuda", dtype)
+    ident = torch.arange(n, dtype=torch.int32, device="cuda")
+    for m in (pack._map_fwd(), pack._map_bwd()):
+        for b, relu in ((None, False), (bias, True)):
+            y = ext.implicit_gemm(x, a, m, n, None, b, relu)
+            y_skip = ext.implicit_gemm(x, a, m, n, ident, b, relu)
+            assert torch.equal(y, y_skip), (b is not None, relu)
+            assert bool(torch.isfinite(y).all())
+
+
 def test_row_orders_match_oracle():
     """Global and windowed row orders equal the oracle's (a deterministic sort: descending
     tap mask, ties in row order, windows in place), on one block and on many, and the
@@ -489,6 +519,11 @@ def test_automatic_row_order_policy():
     assert p3._order_for(True, 64, f32) is not None and p3._order_for(True, 128, bf) is not None
     assert p2._order_for(True, 64, bf) is None          # coarse rows: 1..8 taps each
     assert p2._order_for(False, 64, bf) is None         # fine rows (one tap each), narrow 16-bit
+    from fvdb.sparse_conv_pack_info import wants_row_order
+    # large 128-channel 3x3x3 layers with 128k outputs stay in tree order (direct-to-LDS kernel)
+    assert not wants_row_order(27, 128, bf, 128, 300_000) and not wants_row_order(27, 128, bf, 256, 200_000)
+    assert wants_row_order(27, 128, bf, 128, 100_000) and wants_row_order(27, 128, bf, 64, 300_000)
+    assert wants_row_order(27, 128, f32, 128, 300_000) and wants_row_order(8, 128, bf, 128, 300_000)
     o = p2._order_for(False, 64, f32)                  # fine rows in fp32: windowed order
     assert o is not None
     assert np.array_equal(o.cpu().numpy(), oracle.row_order(p2._map_bwd()[:grid.total_voxels].cpu().numpy(), 8192))
@@ -502,7 +537,7 @@ def test_automatic_row_order_policy():
             outs = []
             for auto in (True, False):
                 if not auto:
-                    pack._order_for = lambda fwd, c, dt: None  # instance attribute shadows the method
+                    pack._order_for = lambda fwd, c, dt, co=0: None  # instance attribute shadows the method
                 try:
                     xx, ww = x.clone().requires_grad_(True), w.clone().requires_grad_(True)
                     f = pack.sparse_transpose_conv_3d if transposed else pack.sparse_conv_3d
