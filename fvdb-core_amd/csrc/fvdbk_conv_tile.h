@@ -467,9 +467,9 @@ __device__ inline void wait_vm_lds(int n) {
 // each gathered row feeds twice the MFMAs and layers of 128k output channels gather
 // every row half as often; 16 KB weight tiles, gathers LX = 2 taps ahead, and the
 // output leaves in two 64-channel halves through slices of the ring.
-// NCC = 2 (128 input channels, with MI = 8): each pipeline step is one (tap, 64-channel
-// chunk) pair, chunk fastest -- the accumulation order of k_conv_tile_c, so results are
-// bit-identical to it -- with one step of gather lookahead (54 steps for 3x3x3).
+// NCC = 2 (128 input channels): each pipeline step is one (tap, 64-channel chunk) pair,
+// chunk fastest -- the accumulation order of k_conv_tile_c, so results are bit-identical
+// to it (54 steps for 3x3x3).
 // Features must span fewer than 2^31 bytes (32-bit buffer offsets); callers check.
 // Weight rings of k_conv_tile_dma: three separate LDS objects of 8 KB (MI = 4) or 16 KB
 // (MI = 8) (file scope: a static __shared__ array inside a kernel template does not link;
@@ -495,7 +495,7 @@ void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict_
     static_assert(NW == 4 || NW == 8, "4 or 8 wavefronts");
     static_assert(MI == 4 || MI == 8, "64 or 128 output channels");
     static_assert(KV > LX, "more steps than the gather lookahead");
-    static_assert(NCC == 1 || MI == 8, "wide inputs take 128-channel workgroups");
+    static_assert(NCC == 1 || NCC == 2, "64 or 128 input channels");
     static_assert(OUT_SLICE <= WT, "output slices fit the weight buffers");
     static_assert(MI == 4 || 3 * OUT_SLICE <= WT, "three slices per 16 KB buffer");
     static_assert(WT * sizeof(T) == (MI == 4 ? sizeof(dma_ring0) : sizeof(dma8_ring0)), "ring buffer size");
@@ -664,24 +664,38 @@ static bool launch_tile_dma(const T* X, int64_t n_in, const T* A, const int32_t*
     return true;
 }
 
-// Direct-to-LDS contraction for 16-bit 3x3x3 layers with 128 input channels and 128k output
-// channels in tree order (NCC = 2, 128-channel workgroups of 8 wavefronts, one step of
-// lookahead); large layers only (>= 1024 such tiles), false otherwise.  Planes + spheres
-// 128 -> 128 (1.1M rows): 0.71 ms against 1.04 ms for the tap-skipping kernel over rows in
-// tap-mask order and 1.74 ms for the register-staged kernel in tree order (MI355X).
+// Direct-to-LDS contraction for 16-bit 3x3x3 layers with 128 input channels in tree order
+// (NCC = 2, workgroups of 8 wavefronts); large layers only (>= 1024 such tiles), false
+// otherwise.  Outputs in multiples of 128 channels take 128-channel workgroups one step
+// ahead, other multiples of 64 take 64-channel workgroups three steps ahead.  Planes +
+// spheres (1.1M rows), MI355X: 128 -> 128 0.71 ms against 1.04 ms for the tap-skipping
+// kernel over rows in tap-mask order and 1.74 ms for the register-staged kernel in tree
+// order; 128 -> 64 0.51 ms against 0.61-0.67 ms.
 template <typename T>
 static bool launch_tile_dma_wide(const T* X, int64_t n_in, const T* A, const int32_t* map, int64_t n_rows, int K,
                                  int M, T* Y, const T* bias, int relu, hipStream_t s) {
-    if (K != 27 || M % 128 != 0 || n_in * 128 * (int64_t)sizeof(T) > 0x7FFFFF00 || ((uintptr_t)Y % 16) != 0 ||
-        (n_rows + 255) / 256 * (M / 128) < 1024)
+    if (K != 27 || M % 64 != 0 || n_in * 128 * (int64_t)sizeof(T) > 0x7FFFFF00 || ((uintptr_t)Y % 16) != 0)
         return false;
     constexpr int NW = 8, KT = 27;
-    const int n_mt = M / 128;
+    const size_t map_bytes = (size_t)32 * NW * KT * 4;
+    if (M % 128 == 0) {
+        if ((n_rows + 255) / 256 * (M / 128) < 1024) return false;
+        const int n_mt = M / 128;
+        const int64_t n_tiles = (n_rows + 32 * NW - 1) / (32 * NW) * n_mt;
+        const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
+        // dynamic LDS: the map tile (output slices live in the ring)
+        hipLaunchKernelGGL((k_conv_tile_dma<T, 1, NW, KT, 8, 2>), grid, dim3(64 * NW), map_bytes, s, X, n_in, A, map,
+                           n_rows, M, n_mt, n_tiles, Y, bias, relu);
+        return true;
+    }
+    if ((n_rows + 255) / 256 * (M / 64) < 1024) return false;
+    const int n_mt = M / 64;
     const int64_t n_tiles = (n_rows + 32 * NW - 1) / (32 * NW) * n_mt;
     const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
-    const size_t lds = (size_t)32 * NW * KT * 4;  // map tile (output slices live in the ring)
-    hipLaunchKernelGGL((k_conv_tile_dma<T, 1, NW, KT, 8, 2>), grid, dim3(64 * NW), lds, s, X, n_in, A, map, n_rows, M,
-                       n_mt, n_tiles, Y, bias, relu);
+    const size_t out_bytes = (size_t)(NW - 3) * 2 * 16 * (64 + 8) * sizeof(T);
+    hipLaunchKernelGGL((k_conv_tile_dma<T, 3, NW, KT, 4, 2>), grid, dim3(64 * NW),
+                       map_bytes > out_bytes ? map_bytes : out_bytes, s, X, n_in, A, map, n_rows, M, n_mt, n_tiles, Y,
+                       bias, relu);
     return true;
 }
 

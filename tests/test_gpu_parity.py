@@ -440,9 +440,10 @@ def test_direct_lds_tile_bit_exact(dtype, cout, large, ks, st):
 
 
 @pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
-def test_direct_lds_wide_bit_exact(dtype):
+@pytest.mark.parametrize("cout", [256, 192])
+def test_direct_lds_wide_bit_exact(dtype, cout):
     """The direct-to-LDS contraction for 128 input channels (steps of one tap and one
-    64-channel chunk, 128-channel workgroups) equals the tap-skipping kernel over an
+    64-channel chunk; 128-channel workgroups for 256 outputs, 64-channel ones for 192) equals the tap-skipping kernel over an
     identity row order bit for bit, with and without the fused bias + ReLU, on a layer
     large enough to take it (>= 1024 workgroups of 256 rows), forward and as dgrad
     (transposed map)."""
@@ -453,8 +454,8 @@ def test_direct_lds_wide_bit_exact(dtype):
     grid = fvdb.GridBatch.from_points(sphere_points(seed=5, n_points=1_500_000, device="cuda"), voxel_sizes=1 / 512)
     pack, _ = grid.sparse_conv_kernel_map(3, 1)
     pack.build_implicit_gemm(training=True)
-    n, cout = grid.total_voxels, 256
-    assert (n + 255) // 256 * (cout // 128) >= 1024, n
+    n = grid.total_voxels
+    assert (n + 255) // 256 * (cout // 128 if cout % 128 == 0 else cout // 64) >= 1024, n
     print(f"[direct-lds-wide] {dtype} rows={n} (last tile {n % 256} rows)", flush=True)
     g = torch.Generator().manual_seed(22)
     x = torch.randn(n, 128, generator=g).to("cuda", dtype)
@@ -522,7 +523,8 @@ def test_automatic_row_order_policy():
     from fvdb.sparse_conv_pack_info import wants_row_order
     # large 128-channel 3x3x3 layers with 128k outputs stay in tree order (direct-to-LDS kernel)
     assert not wants_row_order(27, 128, bf, 128, 300_000) and not wants_row_order(27, 128, bf, 256, 200_000)
-    assert wants_row_order(27, 128, bf, 128, 100_000) and wants_row_order(27, 128, bf, 64, 300_000)
+    assert not wants_row_order(27, 128, bf, 64, 300_000) and wants_row_order(27, 128, bf, 64, 200_000)
+    assert wants_row_order(27, 128, bf, 128, 100_000) and wants_row_order(27, 128, bf, 96, 300_000)
     assert wants_row_order(27, 128, f32, 128, 300_000) and wants_row_order(8, 128, bf, 128, 300_000)
     o = p2._order_for(False, 64, f32)                  # fine rows in fp32: windowed order
     assert o is not None
