@@ -603,6 +603,11 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
             const uint32_t add = (uint32_t)(o[0] - 8 * lo[0] + of[0]) | ((uint32_t)(o[1] - 8 * lo[1] + of[1]) << 8) |
                                  ((uint32_t)(o[2] - 8 * lo[2] + of[2]) << 16);
             const uint32_t leaf_w = (uint32_t)(ext[1] * ext[2]) | ((uint32_t)ext[2] << 8) | (1u << 16);
+            // stores through a buffer resource over the leaf's block: rows past the leaf and
+            // idle lanes take an offset beyond it, which the hardware drops, so no branch
+            const __amdgpu_buffer_rsrc_t ob =
+                __builtin_amdgcn_make_buffer_rsrc((void*)out, (short)0, cnt * K * 4, 0x00020000);
+            const uint32_t lane_off = act ? (uint32_t)lk * 4u : 0x80000000u;
             for (int r0 = lr; r0 < cnt; r0 += rpi * U) {
                 uint32_t P[U];
 #pragma unroll
@@ -625,7 +630,7 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
                     const uint32_t m = mb[u].x;
                     // index if the probe bit is set, else all ones (-1), without a branch
                     const uint32_t idx = (mb[u].y + __popc(m & ((1u << sh[u]) - 1u))) | (((m >> sh[u]) & 1u) - 1u);
-                    if (act && rr < cnt) out[(uint32_t)(rr * K + lk)] = (int32_t)idx;
+                    __builtin_amdgcn_raw_buffer_store_b32(idx, ob, lane_off + (uint32_t)(rr * K * 4), 0, 0);
                 }
             }
             return;
