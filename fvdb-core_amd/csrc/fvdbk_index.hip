@@ -635,6 +635,9 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
             }
             return;
         }
+        // branch-free stores, as in the stride-1 loop above
+        const __amdgpu_buffer_rsrc_t ob = __builtin_amdgcn_make_buffer_rsrc((void*)out, (short)0, cnt * K * 4, 0x00020000);
+        const uint32_t lane_off = act ? (uint32_t)lk * 4u : 0x80000000u;
         for (int r0 = lr; r0 < cnt; r0 += rpi * U) {
             uint32_t vox[U];
 #pragma unroll
@@ -657,8 +660,8 @@ __global__ __launch_bounds__(64 * KMAP_WAVES) void k_kmap(fvdbk_grid rows, fvdbk
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int rr = r0 + rpi * u;
-                if (act && rr < cnt)
-                    out[(int64_t)rr * K + lk] = staged_index(ok[u], q[u], mb[u].x, (int32_t)mb[u].y, sh[u]);
+                const int32_t idx = staged_index(ok[u], q[u], mb[u].x, (int32_t)mb[u].y, sh[u]);
+                __builtin_amdgcn_raw_buffer_store_b32((uint32_t)idx, ob, lane_off + (uint32_t)(rr * K * 4), 0, 0);
             }
         }
         return;
