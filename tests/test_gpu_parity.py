@@ -5,9 +5,9 @@ pairs) must be bit-exact.  Floating-point tolerances, stated per test and measur
 max|gpu - ref| / max|ref| against the float64 oracle:
 
 * fp64 on the GPU: 1e-12
-* fp32:            1e-5   (the north-star bound)
+* fp32:            1e-5   (the required bound)
 * fp16:            2e-3
-* bf16:            1e-2   (the north-star bound)
+* bf16:            1e-2   (the required bound)
 """
 import math
 import os
