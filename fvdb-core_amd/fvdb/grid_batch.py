@@ -82,6 +82,10 @@ def _full_jidx(j: JaggedTensor) -> torch.Tensor:
     return j.jidx.to(torch.int32)
 
 
+def _grid_from_state(ijk, jidx, n_grids, voxel_sizes, origins) -> "GridBatch":
+    return GridBatch(impl=_build_index(ijk, jidx, int(n_grids), voxel_sizes.clone(), origins.clone()))
+
+
 class GridBatch:
     def __init__(self, *, impl: _GridIndex):
         self._impl = impl
@@ -378,6 +382,17 @@ class GridBatch:
 
     def cuda(self) -> "GridBatch":
         return self.to("cuda")
+
+    def __reduce__(self):
+        """Pickle support, as ``GridBatchBinding.cpp:890-894`` gives the reference's grids.
+
+        The state is the voxel coordinates in linear order with their grid indices, the grid count
+        and the per-grid voxel sizes and origins; unpickling rebuilds the index on the device the
+        coordinates were saved from, in the same voxel order.  The pickled bytes are this
+        package's own layout, not a NanoVDB buffer.
+        """
+        i = self._impl
+        return (_grid_from_state, (i.ijk, i.jidx, i.n_grids, i.voxel_sizes, i.origins))
 
     def cpu(self) -> "GridBatch":
         raise ValueError("fvdb-core_amd grids live on HIP devices only")

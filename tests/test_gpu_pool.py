@@ -240,3 +240,26 @@ def test_ijk_to_inv_index_and_inject_from_ijk():
     dst = g.jagged_like(torch.zeros(g.total_voxels, 2, dtype=torch.float64, device="cuda"))
     res = g.inject_from_ijk(sq, sv, dst=dst)
     assert res is dst and np.array_equal(dst.jdata.cpu().numpy(), np.where(want == -5.0, 0.0, want))
+
+
+def test_grid_pickle_round_trip():
+    """pickle.dumps / pickle.loads of a GridBatch gives the same grids: voxel coordinates in the
+    same linear order, the same grid offsets, voxel sizes and origins, and the same answers
+    from ijk_to_index; a single grid and a batch holding an empty grid both round-trip."""
+    import pickle
+    fvdb = _fvdb()
+    rng = np.random.default_rng(31)
+    g, _ = grids(fvdb, rng, [400, 250, 90], 40)
+    g = fvdb.GridBatch.from_ijk(g.ijk, voxel_sizes=[[0.5, 0.25, 1.0], [1.0, 1.0, 1.0], [2.0, 2.0, 0.125]],
+                                origins=[[0.0, 1.0, -2.0], [0.5, 0.5, 0.5], [3.0, 0.0, 0.0]])
+    empty = torch.empty((0, 3), dtype=torch.int32, device="cuda")
+    single = fvdb.GridBatch.from_ijk(g.ijk.jdata[:150].clone())
+    with_empty = fvdb.GridBatch.from_ijk(fvdb.JaggedTensor([g.ijk.jdata[:60].clone(), empty]))
+    for src in (g, single, with_empty):
+        back = pickle.loads(pickle.dumps(src))
+        assert back.grid_count == src.grid_count and back.device == src.device
+        assert torch.equal(back.ijk.jdata, src.ijk.jdata)
+        assert torch.equal(back.joffsets, src.joffsets)
+        assert torch.equal(back.voxel_sizes, src.voxel_sizes) and torch.equal(back.origins, src.origins)
+        q = fvdb.JaggedTensor([src.ijk.jdata.flip(0)] * src.grid_count)
+        assert torch.equal(back.ijk_to_index(q).jdata, src.ijk_to_index(q).jdata)
