@@ -45,11 +45,12 @@ def log(msg: str) -> None:
 
 
 def build_workload(torch, fvdb, device):
-    from fvdb.utils.synthetic import log_uniform, sphere_points
+    from fvdb.utils.synthetic import log_uniform, sphere_points, voxelize
 
     vs = log_uniform(0, 1 / 512, 1 / 128, 8)
     pts = [sphere_points(seed=s + 1, n_points=2_000_000, device=device) for s in range(8)]
-    grid = fvdb.GridBatch.from_points(fvdb.JaggedTensor(pts), voxel_sizes=[[v, v, v] for v in vs], origins=0.0)
+    # origins that put every voxel in the root tile [0, 4096)^3 (checked by voxelize)
+    grid = voxelize(pts, vs)
     del pts
     g = torch.Generator(device=device).manual_seed(1234)
     N = grid.total_voxels
@@ -384,10 +385,10 @@ def run_single(torch, fvdb, dev, args):
     import numpy as np
 
     import oracle
-    from fvdb.utils.synthetic import sphere_points
+    from fvdb.utils.synthetic import sphere_points, voxelize
 
     pts = sphere_points(seed=0, n_points=2_000_000, device=dev)
-    grid = fvdb.GridBatch.from_points(pts, voxel_sizes=1 / 256)
+    grid = voxelize(pts, 1 / 256)
     N = grid.total_voxels
     g = torch.Generator(device=dev).manual_seed(0)
     x = torch.randn((N, 32), generator=g, device=dev)
@@ -427,10 +428,10 @@ def run_single(torch, fvdb, dev, args):
 
 def run_diff(torch, fvdb, dev, args):
     """~1M voxels (planes + spheres at 1/1024), 3x3x3 128->128 bf16 forward + dgrad + wgrad."""
-    from fvdb.utils.synthetic import planes_spheres_points
+    from fvdb.utils.synthetic import planes_spheres_points, voxelize
 
     pts = planes_spheres_points(seed=7, n_points=2_400_000, n_planes=2, n_spheres=2, device=dev)
-    grid = fvdb.GridBatch.from_points(pts, voxel_sizes=1 / 1024)
+    grid = voxelize(pts, 1 / 1024)
     N = grid.total_voxels
     pack, _ = grid.sparse_conv_kernel_map(3, 1)
     pack.build_implicit_gemm(training=True)
@@ -535,7 +536,7 @@ def run_orders(torch, fvdb, dev, args):
     pick by themselves where ``wants_row_order`` says so), bf16 and fp32, on the precision, differentiation
     and U-Net grids; outputs are checked bit-identical between the two orders."""
     from fvdb.sparse_conv_pack_info import AUTO_ORDER_WINDOW, wants_row_order
-    from fvdb.utils.synthetic import planes_spheres_points
+    from fvdb.utils.synthetic import planes_spheres_points, voxelize
 
     ext = fvdb.ext
     rows = []
@@ -564,7 +565,7 @@ def run_orders(torch, fvdb, dev, args):
     case("sphere batch 3x3x3 64->64", pa._map_fwd(), grid.total_voxels, grid.total_voxels, 64, 64)
     del grid, pa
     pts = planes_spheres_points(seed=7, n_points=2_400_000, n_planes=2, n_spheres=2, device=dev)
-    gd = fvdb.GridBatch.from_points(pts, voxel_sizes=1 / 1024)
+    gd = voxelize(pts, 1 / 1024)
     pd, _ = gd.sparse_conv_kernel_map(3, 1)
     case("planes+spheres 3x3x3 128->128", pd._map_fwd(), gd.total_voxels, gd.total_voxels, 128, 128)
     del gd, pd
@@ -634,14 +635,14 @@ def run_serve(torch, fvdb, dev, args):
     """Poisson stream of single grids (log-uniform 10k-2M voxels), the U-Net in bf16 with continuous
     batching, against a one-at-a-time fp32 loop over the same stream."""
     from fvdb.serving import ContinuousBatcher, Request, UNetExecutor, latency_stats
-    from fvdb.utils.synthetic import log_uniform, sphere_points_for_voxels
+    from fvdb.utils.synthetic import log_uniform, sphere_points_for_voxels, voxelize
 
     nreq = args.requests
     sizes = [int(v) for v in log_uniform(42, 10_000, 2_000_000, nreq)]
     reqs_ijk = []
     for i, nv in enumerate(sizes):
         pts, vs = sphere_points_for_voxels(1000 + i, nv, device=dev)
-        g = fvdb.GridBatch.from_points(pts, voxel_sizes=vs)
+        g = voxelize(pts, vs)
         reqs_ijk.append(g.ijk.jdata.cpu().pin_memory())
         del pts, g
     counts = [t.shape[0] for t in reqs_ijk]

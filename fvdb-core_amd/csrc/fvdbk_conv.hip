@@ -155,11 +155,26 @@ static int conv_mfma_dispatch(const T* X, int64_t n_in, int cin, const T* A, con
 // gw[k][c][o] = sum_p in[pi][c] * gout[po][o].  A workgroup takes one chunk of the
 // pairs of one offset and a (c, o) tile; rows of both operands are gathered into LDS
 // (16-byte loads), then read back as MFMA fragments whose reduction index is the
-// pair: lane group g, element e takes pair row  step*CH + 4e + g  of the stage (the
-// same assignment for A and B, so every pair meets itself).  Partials are fp32.
+// pair: for 16-bit types lane group g, element e takes pair row  step*CH + 8g + e  of
+// the stage through LDS transpose reads (two 8-byte reads per fragment); for fp32
+// element e takes row  step*CH + 4e + g.  A and B use the same assignment, so every
+// pair meets itself.  Partials are fp32.
 // ---------------------------------------------------------------------------------
 
 constexpr int WG_ROWS = 64;  // pairs staged per LDS round
+
+// Segment swizzle of the 16-bit wgrad tiles (rows of NSEG 16-byte segments): the lanes
+// of one transpose read cover rows st*32 + 8g + (0..3) of a stage, 8 consecutive banks
+// per row; the swizzle puts the rows read by each half-wavefront in distinct 8-bank
+// groups.  It is even (a lane's two segments stay adjacent) and depends on row bits 0,
+// 1 and 3 only, so rows r and r + 4 share it.
+template <int NSEG>
+__device__ __host__ constexpr int wg_swizzle(int r) {
+    return NSEG >= 16 ? 2 * ((r & 3) | (((r >> 3) & 1) << 2))
+           : NSEG == 8 ? 2 * (((r >> 1) & 1) | (((r >> 3) & 1) << 1))
+           : NSEG == 4 ? 2 * ((r >> 3) & 1)
+                       : 0;
+}
 
 // Software pipeline: each thread owns fixed (pair row, channel group)
 // slots of the staged tiles.  While round r is contracted from LDS, the rows of round
@@ -177,7 +192,10 @@ __global__ __launch_bounds__(256) void k_wgrad_pipe(const T* __restrict__ X, int
     using V = typename F::vec;
     constexpr int TM = 2 * MI * 16;
     constexpr int TN = 2 * NI * 16;
-    constexpr int PADB = 32;
+    // 16-bit tiles: unpadded rows, 16-byte segment s of row r stored at s ^ wg_swizzle(r)
+    // (bank-conflict-free transpose reads); fp32 tiles: rows padded by 32 bytes
+    constexpr bool TR = sizeof(T) == 2;
+    constexpr int PADB = TR ? 0 : 32;
     constexpr int XP = TM + PADB / (int)sizeof(T);
     constexpr int GP = TN + PADB / (int)sizeof(T);
     constexpr int XSEG = TM / F::EPL, GSEG = TN / F::EPL;
@@ -251,15 +269,15 @@ __global__ __launch_bounds__(256) void k_wgrad_pipe(const T* __restrict__ X, int
     auto store_rows = [&]() {
 #pragma unroll
         for (int q = 0; q < XPT; ++q) {
-            const int e = tid + 256 * q;
-            if (e < WG_ROWS * XSEG)
-                *reinterpret_cast<V*>(&sX[(e / XSEG) * XP + (e % XSEG) * F::EPL]) = xv[q];
+            const int e = tid + 256 * q, r = e / XSEG;
+            const int seg = TR ? (e % XSEG) ^ wg_swizzle<XSEG>(r) : e % XSEG;
+            if (e < WG_ROWS * XSEG) *reinterpret_cast<V*>(&sX[r * XP + seg * F::EPL]) = xv[q];
         }
 #pragma unroll
         for (int q = 0; q < GPT; ++q) {
-            const int e = tid + 256 * q;
-            if (e < WG_ROWS * GSEG)
-                *reinterpret_cast<V*>(&sG[(e / GSEG) * GP + (e % GSEG) * F::EPL]) = gv[q];
+            const int e = tid + 256 * q, r = e / GSEG;
+            const int seg = TR ? (e % GSEG) ^ wg_swizzle<GSEG>(r) : e % GSEG;
+            if (e < WG_ROWS * GSEG) *reinterpret_cast<V*>(&sG[r * GP + seg * F::EPL]) = gv[q];
         }
     };
 
@@ -278,14 +296,33 @@ __global__ __launch_bounds__(256) void k_wgrad_pipe(const T* __restrict__ X, int
 #pragma unroll
         for (int st = 0; st < WG_ROWS / F::CH; ++st) {
             V a[MI], b[NI];
+            if constexpr (TR) {
+                // element e of lane group g is pair row st*CH + 8g + e: two transpose reads,
+                // rows tr and tr + 4 (the same swizzle: it depends on bits 0-1 and 3 of the row)
+                const int tr = st * F::CH + 8 * g + (r16 >> 2);
+                const int half = (r16 & 1) * 4;  // element offset inside the 16-byte segment
+                const int sx = wg_swizzle<XSEG>(tr), sg = wg_swizzle<GSEG>(tr);
 #pragma unroll
-            for (int i = 0; i < MI; ++i)
+                for (int i = 0; i < MI; ++i) {
+                    const int seg = ((wm + i * 16) / 8 + ((r16 & 3) >> 1)) ^ sx;
+                    a[i] = lds_frag_tr16<V>(&sX[tr * XP + seg * 8 + half], XP);
+                }
 #pragma unroll
-                for (int e = 0; e < F::EPL; ++e) a[i][e] = sX[(st * F::CH + 4 * e + g) * XP + wm + i * 16 + r16];
+                for (int j = 0; j < NI; ++j) {
+                    const int seg = ((wn + j * 16) / 8 + ((r16 & 3) >> 1)) ^ sg;
+                    b[j] = lds_frag_tr16<V>(&sG[tr * GP + seg * 8 + half], GP);
+                }
+            } else {
+                // element e of lane group g is pair row st*CH + 4e + g
 #pragma unroll
-            for (int j = 0; j < NI; ++j)
+                for (int i = 0; i < MI; ++i)
 #pragma unroll
-                for (int e = 0; e < F::EPL; ++e) b[j][e] = sG[(st * F::CH + 4 * e + g) * GP + wn + j * 16 + r16];
+                    for (int e = 0; e < F::EPL; ++e) a[i][e] = sX[(st * F::CH + 4 * e + g) * XP + wm + i * 16 + r16];
+#pragma unroll
+                for (int j = 0; j < NI; ++j)
+#pragma unroll
+                    for (int e = 0; e < F::EPL; ++e) b[j][e] = sG[(st * F::CH + 4 * e + g) * GP + wn + j * 16 + r16];
+            }
 #pragma unroll
             for (int i = 0; i < MI; ++i)
 #pragma unroll
@@ -348,7 +385,7 @@ __global__ void k_wgrad_reduce(const Acc* __restrict__ ws, const int32_t* __rest
         const Acc x = ws[(int64_t)c * per + e];
         const Acc t = sum + x;
         if (sizeof(Acc) == 4) {
-            if (fabs((double)sum) >= fabs((double)x)) comp += (sum - t) + x;
+            if (fabsf((float)sum) >= fabsf((float)x)) comp += (sum - t) + x;
             else comp += (x - t) + sum;
         }
         sum = t;

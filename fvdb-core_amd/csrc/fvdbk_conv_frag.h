@@ -71,6 +71,32 @@ __device__ inline typename Frag<T>::vec load_frag(const T* __restrict__ p, int c
     return v;
 }
 
+// LDS transpose read of 16-bit data (gfx950 ds_read_b64_tr_b16).  Within each group of
+// 16 lanes, lane s = 4j + q supplies the address of 4 consecutive elements and lane
+// i = 4q + p receives element p of lane (4j + q)'s elements as its element j.  So when
+// lane s points at row (r0 + s/4), columns (c0 + 4(s%4)) .. +3 of a row-major tile,
+// lane i receives column c0 + i of rows r0 .. r0+3: one read per 4 rows instead of a
+// 2-byte read per row.
+typedef short shortx4 __attribute__((ext_vector_type(4)));
+typedef short shortx8 __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(3))) shortx4 lds_shortx4;
+typedef __attribute__((address_space(3))) short lds_short;
+
+__device__ inline shortx4 lds_read_tr16(const void* p) {
+    return __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_shortx4*)(lds_short*)(const short*)p);
+}
+
+// An 8-element MFMA operand whose element e is column (c0 + lane%16) of tile row
+// (r0 + e), where `p` is this lane's address of row (r0 + (lane%16)/4), column
+// (c0 + 4*(lane%4)) and rows are `row_stride` elements apart.
+template <typename V>
+__device__ inline V lds_frag_tr16(const void* p, int row_stride) {
+    const shortx4 lo = lds_read_tr16(p);
+    const shortx4 hi = lds_read_tr16((const short*)p + 4 * row_stride);
+    const shortx8 v = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_bit_cast(V, v);
+}
+
 template <typename T>
 __device__ inline typename Frag<T>::vec zero_frag() {
     typename Frag<T>::vec v;

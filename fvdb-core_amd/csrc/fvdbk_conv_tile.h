@@ -546,7 +546,10 @@ void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict_
     };
     const int bytes = (int)(n_in * CIN * (int64_t)sizeof(T));
     const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, bytes, 0x00020000);
-    constexpr int MISSING = 0x7FFFFFC0;        // beyond any buffer: the load returns zeros
+    // A missing neighbour's offset: at or beyond the buffer's size (features span at most
+    // 0x7FFFFF00 bytes, checked by the launchers), so the load returns zeros, and with
+    // room below 2^31 for the chunk offsets added to it (at most 64 bytes).
+    constexpr int MISSING = 0x7FFFFF00;
     V b[LX + 1][NI][NCH];
     auto gather = [&](int v, V (&bb)[NI][NCH]) {
         const int k = v / NCC, c = v % NCC;

@@ -375,6 +375,8 @@ class GridBatch:
         dev = target.device if hasattr(target, "device") else torch.device(target)
         if dev.type != "cuda":
             raise ValueError("fvdb-core_amd grids live on HIP devices only")
+        if dev.index is None:  # "cuda" means the current device
+            dev = torch.device("cuda", torch.cuda.current_device())
         if dev == self.device:
             return self
         return GridBatch(impl=_build_index(self._impl.ijk.to(dev), self._impl.jidx.to(dev), self.grid_count,
@@ -545,24 +547,42 @@ class GridBatch:
     # ----------------------------------------------------------------------------
     # dense I/O (differentiable; used by the dense-equivalence tests)
     # ----------------------------------------------------------------------------
+    def _coords_per_grid(self, v, name: str) -> torch.Tensor:
+        """[B, 3] int64 from a scalar, a [3] or a [B, 3] integer value (one coordinate per grid)."""
+        t = torch.as_tensor(v, dtype=torch.int64, device=self.device)
+        if t.numel() == 1:
+            t = t.reshape(1).expand(3)
+        t = t.reshape(-1, 3) if t.numel() % 3 == 0 else t
+        if t.dim() != 2 or t.shape[0] not in (1, self.grid_count):
+            raise ValueError(f"{name} must have shape [3,] or [B, 3]")
+        return t.expand(self.grid_count, 3)
+
     def _dense_frame(self, min_coord, grid_size):
+        """Origin [B, 3] and size of the dense frame, as ``autograd/ReadIntoDense.cpp:31-56``
+        chooses them: the origin is `min_coord` (one per grid or broadcast) or the minimum of
+        the batch's total bounding box; the size is `grid_size` or the total bounding box's
+        extent (max - min + 1), whatever `min_coord` is."""
+        bbox = self.total_bbox.to(torch.int64)
         if min_coord is None:
-            lo = self.total_bbox[0].to(torch.int64).reshape(1, 3).expand(self.grid_count, 3)
+            lo = bbox[0].reshape(1, 3).expand(self.grid_count, 3)
         else:
-            lo = torch.as_tensor(min_coord, dtype=torch.int64, device=self.device).reshape(-1, 3)
-            lo = lo.expand(self.grid_count, 3)
+            lo = self._coords_per_grid(min_coord, "min_coord")
         if grid_size is None:
-            hi = self.total_bbox[1].to(torch.int64)
-            size = (hi - lo.min(0).values + 1).clamp(min=0)
-            size = tuple(int(v) for v in size.tolist())
+            size = tuple(int(v) for v in (bbox[1] - bbox[0] + 1).tolist())
         else:
             size = to_vec3i(grid_size, "grid_size", positive=True)
         return lo, size
 
     def write_to_dense_xyzc(self, sparse_data, min_coord=None, grid_size=None) -> torch.Tensor:
+        """Dense [B, X, Y, Z, *] tensor of the voxels' rows (zeros elsewhere); voxel ijk of grid b
+        lands at ijk - min_coord[b] when inside the frame (``autograd/ReadIntoDense.cpp:20-80``)."""
         data = sparse_data.jdata if isinstance(sparse_data, JaggedTensor) else sparse_data
+        if data.dim() < 2:
+            raise ValueError("sparse_data must have shape [num_voxels, *]")
         if data.shape[0] != self.total_voxels:
-            raise ValueError("sparse_data must have one row per voxel")
+            raise ValueError(f"sparseData must have shape (num_voxels, *) where num_voxels = {self.total_voxels}")
+        if self.total_voxels == 0:
+            raise RuntimeError("Empty grid")
         lo, size = self._dense_frame(min_coord, grid_size)
         b = self._impl.jidx.long()
         rel = self._impl.ijk.long() - lo[b]
@@ -573,13 +593,19 @@ class GridBatch:
         return out.index_put((b[sel], r[:, 0], r[:, 1], r[:, 2]), data[sel])
 
     def write_to_dense_czyx(self, sparse_data, min_coord=None, grid_size=None) -> torch.Tensor:
+        """[B, C, Z, Y, X] layout of ``write_to_dense_xyzc`` (one channel dimension)."""
         d = self.write_to_dense_xyzc(sparse_data, min_coord, grid_size)
-        if d.dim() == 4:
-            d = d.unsqueeze(-1)
+        d = d.reshape(d.shape[:4] + (-1,))
         return d.permute(0, 4, 3, 2, 1)
 
     def read_from_dense_xyzc(self, dense_data: torch.Tensor, dense_origins=0) -> JaggedTensor:
-        lo = torch.as_tensor(dense_origins, dtype=torch.int64, device=self.device).reshape(-1, 3)
+        """Rows [N, *] read from a dense [B, X, Y, Z, *] tensor at ijk - dense_origins[b]; voxels
+        outside it read zeros (``autograd/ReadFromDense.cpp:18-60``)."""
+        if dense_data.dim() <= 4 or dense_data.shape[0] != self.grid_count:
+            raise ValueError("dense data must have shape [B, W, H, D, *]")
+        if self.total_voxels == 0:
+            raise RuntimeError("Empty grid")
+        lo = self._coords_per_grid(dense_origins, "dense_origins")
         lo = lo.expand(self.grid_count, 3)
         b = self._impl.jidx.long()
         rel = self._impl.ijk.long() - lo[b]
@@ -593,6 +619,9 @@ class GridBatch:
         return self.jagged_like(out)
 
     def read_from_dense_czyx(self, dense_data: torch.Tensor, dense_origins=0) -> JaggedTensor:
+        """``read_from_dense_xyzc`` of a dense [B, C, Z, Y, X] tensor."""
+        if dense_data.dim() != 5:
+            raise ValueError("dense data must have shape [B, C, D, H, W]")
         return self.read_from_dense_xyzc(dense_data.permute(0, 4, 3, 2, 1), dense_origins)
 
     # ----------------------------------------------------------------------------
@@ -628,8 +657,14 @@ class GridBatch:
             raise ValueError("Batch size not match!")
         if variant not in (8, 64):
             raise ValueError("variant must be 8 or 64")
+        if not weight.is_floating_point():
+            raise TypeError("kernels must have a floating point type")
+        if weight.dim() != 5:
+            raise ValueError(f"Expected kernels to have 5 dimensions (shape (out_ch, in_ch, d, h, w)) but got "
+                             f"{weight.dim()} dimensions")
         if tuple(weight.shape[2:]) != (3, 3, 3):
-            raise ValueError("halo convolution needs a 3x3x3 kernel")
+            # TORCH_CHECK_NOT_IMPLEMENTED in autograd/SparseConvolutionHalo.cpp:30-32
+            raise NotImplementedError("sparse_conv_halo only supports kernels of size 3x3x3")
         if self._halo_pack is None:
             self._halo_pack = SparseConvPackInfo((3, 3, 3), (1, 1, 1), self, None)
         out = sparse_conv(j.jdata, weight, self._halo_pack, False)

@@ -5,8 +5,11 @@ Requests are single grids (voxel coordinates plus features) that arrive over tim
 * **Ingest** (a separate HIP stream): each admitted request's grid index and all the kernel maps its network
   needs (one 3x3x3 map per level, the stride-2 down map and its transpose between levels) are built once with
   the gfx950 kernels and stay resident on the device with the request.
-* **Admission**: a request is admitted when its estimated device memory (maps, features, activations, output)
-  fits in the remaining HBM budget; reservations are released when its batch completes or fails.
+* **Admission**: a request is admitted (FIFO) when an upper bound of its device memory (maps, features,
+  activations, output, ingest buffers) fits in the remaining HBM budget. Before ingest only the input voxel count
+  is known, so the bound takes every level as large as the input; once the request is ingested its level sizes
+  are known and the reservation drops to the bound for those sizes (``exact_bytes``). Reservations are released
+  when the request's batch completes or fails.
 * **Batching**: ready requests are packed (FIFO, up to a voxel cap) into one jagged launch per layer. The
   batch's kernel maps are the per-request maps concatenated with their indices shifted by the request's row
   offset at that level, so no grid is rebuilt per batch. Launches go to the compute stream while the next
@@ -16,7 +19,7 @@ Requests are single grids (voxel coordinates plus features) that arrive over tim
   are per row. A request's output is therefore bit-identical alone and in any mix.
 
 The scheduler (``ContinuousBatcher``) only talks to an executor through ``estimate_bytes``, ``prepare``,
-``launch``, ``is_done`` and ``finish``, so its admission and failure handling can be exercised with a CPU
+``launch``, ``is_done``, ``finish`` and optionally ``exact_bytes``, so its admission and failure handling can be exercised with a CPU
 executor. ``UNetExecutor`` is the GPU executor for ``fvdb.nn.SparseUNet``.
 """
 from __future__ import annotations
@@ -119,6 +122,13 @@ class ContinuousBatcher:
             except Exception as e:  # noqa: BLE001 - a bad request fails alone
                 self._fail([r], e)
                 continue
+            # once ingested, the request's exact footprint is known: give back the slack of the
+            # admission bound (never more than was reserved)
+            exact = getattr(self.ex, "exact_bytes", None)
+            if exact is not None:
+                need = min(int(exact(r)), r.reserved)
+                self.reserved -= r.reserved - need
+                r.reserved = need
             r.status = "prepared"
             self.prepared.append(r)
 
@@ -248,7 +258,8 @@ def _shift_cat(maps: Sequence[torch.Tensor], offsets: Sequence[int]) -> torch.Te
 class UNetExecutor:
     """Runs a SparseUNet over batches of requests with device-resident per-request maps."""
 
-    def __init__(self, model, dtype=torch.bfloat16, device="cuda", in_channels: Optional[int] = None):
+    def __init__(self, model, dtype=torch.bfloat16, device="cuda", in_channels: Optional[int] = None,
+                 host_results: bool = False):
         import fvdb
 
         from ._autograd import weight_to_kio
@@ -256,6 +267,7 @@ class UNetExecutor:
         self.fvdb = fvdb
         self.device = torch.device(device)
         self.dtype = dtype
+        self.host_results = bool(host_results)  # finish() copies each result to host memory
         self.model = model
         self.levels = len(model.channels)
         self.cin = in_channels or model.enc[0][0].in_channels
@@ -294,16 +306,41 @@ class UNetExecutor:
     def voxels(self, req: Request) -> int:
         return int(req.state["n0"]) if isinstance(req.state, dict) else int(req.ijk.shape[0])
 
-    def estimate_bytes(self, req: Request) -> int:
-        # upper bound before ingest: N0 input voxels; coarse levels at most N0 each (stride-2 surfaces shrink ~4x)
-        n = int(req.ijk.shape[0])
+    def _footprint(self, n_levels: Sequence[int], n0: int) -> int:
+        """Device bytes a request with `n_levels[l]` voxels at level l holds from ingest to
+        completion, an upper bound over the executor's allocations:
+
+        * ingest (transient): the grid builds of two consecutive levels, <= 160 B per input
+          voxel each (coordinates, leaf keys, the de-duplication set, masks, hash), and the
+          generated features in fp32 before their cast;
+        * per level: the 3x3x3 map ([pad128(n_l), 27] int32), the stride-2 map and its
+          transpose ([pad128(n), 8] int32 each), each once more in a batch's concatenated
+          copy, plus a row order (int32) per map;
+        * activations in the feature dtype: the level's skip (c_l), and at most 5 c_l more
+          at once (up-conv output, the 2 c_l concatenation, a convolution output);
+        * the input features (cin, twice: request and batch copy) and the output (c_0).
+        """
         el = torch.tensor([], dtype=self.dtype).element_size()
-        per_voxel = 0
+        pad = lambda v: (int(v) + 127) // 128 * 128  # noqa: E731
+        total = (320 + 4 * self.cin) * n0 + 2 * n0 * self.cin * el + n0 * self.chans[0] * el
         for l, c in enumerate(self.chans):
-            frac = 1.0 / (4 ** l)
-            per_voxel += frac * ((27 + 8 + 8) * 4 + 6 * c * el + 64)
-        per_voxel += self.cin * el + 12
-        return int(n * per_voxel) + (1 << 20)
+            n = int(n_levels[l])
+            nn_ = int(n_levels[l + 1]) if l + 1 < len(self.chans) else 0
+            total += 2 * 4 * (pad(n) * 27 + pad(nn_) * 8 + pad(n) * 8) + 3 * 4 * pad(n)
+            total += n * 6 * c * el
+        return int(total) + (1 << 20)
+
+    def estimate_bytes(self, req: Request) -> int:
+        """Upper bound before ingest.  Only the input voxel count N is known, and a stride-2
+        coarsening keeps at most as many voxels as it is given (every coarse voxel has a fine
+        voxel), so every level is bounded by N."""
+        n = int(req.ijk.shape[0])
+        return self._footprint([n] * self.levels, n)
+
+    def exact_bytes(self, req: Request) -> int:
+        """The same bound with the level sizes of the ingested request."""
+        topo = req.state["topo"]
+        return self._footprint(topo.n, topo.n[0])
 
     def prepare(self, req: Request) -> None:
         fvdb = self.fvdb
@@ -381,4 +418,7 @@ class UNetExecutor:
 
     def finish(self, handle) -> List[torch.Tensor]:
         o, out = handle["offs"], handle["out"]
+        if self.host_results:
+            host = out.to("cpu")
+            return [host[o[i]:o[i + 1]] for i in range(len(o) - 1)]
         return [out[o[i]:o[i + 1]] for i in range(len(o) - 1)]

@@ -11,7 +11,8 @@ Mirrors the reference's ``fvdb/sparse_conv_pack_info.py:33-374`` and
   ``out_in_map_bwd`` (``:90-148``).  With ``sorted=True`` the reference's sort
   metadata (bitmask per split, descending sort -> ``reorder_loc``, reordered map,
   OR-reduced masks) is produced with the same definitions; the order of equal masks is
-  unspecified there (``torch::sort`` is not stable) and here.
+  unspecified there (``torch::sort`` is not stable) and stable here (row order), and with
+  one split ``reorder_loc`` is the order the sorted contraction runs.
 * ``build_cutlass`` / ``build_lggs``: accepted; both backends run on the same gfx950
   implicit-GEMM kernel (their buffers are reported as ``None``).
 * ``sparse_conv_3d`` / ``sparse_transpose_conv_3d`` with the reference's checks
@@ -147,7 +148,9 @@ class SparseConvPackInfo:
         return self._tmap
 
     def _order_fwd(self) -> Optional[torch.Tensor]:
-        """Target rows ordered by tap mask (for the sorted contraction), or None."""
+        """Target rows ordered by tap mask (for the sorted contraction), or None.  For a
+        build with one split this is ``reorder_loc[0][:N_out]``: the order the property
+        reports is the order the contraction runs."""
         if not self._sorted:
             return None
         if self._ord_fwd is None:
@@ -155,7 +158,8 @@ class SparseConvPackInfo:
         return self._ord_fwd
 
     def _order_bwd(self) -> Optional[torch.Tensor]:
-        """Source rows of the transposed map ordered by tap mask, or None."""
+        """Source rows of the transposed map ordered by tap mask, or None (for a training
+        build with one backward split, ``reorder_loc_bwd[0][:N_in]``)."""
         if not self._sorted:
             return None
         if self._ord_bwd is None:
@@ -218,9 +222,11 @@ class SparseConvPackInfo:
         self._use_tf32 = bool(use_tf32)
         self._sorted = bool(sorted)
         if sorted:
-            self._sorted_meta = self._build_sorted_meta(self._kmap, split_mask_num, self._tgt.total_voxels, 128)
+            self._sorted_meta = self._build_sorted_meta(self._kmap, split_mask_num, self._tgt.total_voxels, 128,
+                                                        self._order_fwd())
         if training and sorted:
-            bwd = self._build_sorted_meta(self._tmap, split_mask_num_bwd, self._src.total_voxels, 128)
+            bwd = self._build_sorted_meta(self._tmap, split_mask_num_bwd, self._src.total_voxels, 128,
+                                          self._order_bwd())
             self._sorted_meta.update({
                 "reorder_loc_bwd": bwd["reorder_loc"],
                 "reorder_out_in_map_bwd": bwd["reorder_out_in_map"],
@@ -240,8 +246,16 @@ class SparseConvPackInfo:
             out |= m[:, :, i]
         return out
 
-    def _build_sorted_meta(self, kmap: torch.Tensor, splits: int, n_rows: int, tile: int) -> dict:
-        """Sort metadata of ``IGEMMBitOperations.cu:14-170`` (bitmask, reorder, reduce)."""
+    def _build_sorted_meta(self, kmap: torch.Tensor, splits: int, n_rows: int, tile: int,
+                           rows_order: torch.Tensor) -> dict:
+        """Sort metadata of ``IGEMMBitOperations.cu:14-170`` (bitmask, reorder, reduce).
+
+        The reference sorts each split's bitmask over all padded rows with an unstable
+        descending sort, so the order of equal masks is unspecified there.  Here the sort
+        is stable (equal masks in row order).  With one split the bitmask is the tap mask
+        of ``fvdbk_kmap_row_keys`` (27 taps fit its 31 bits), and ``reorder_loc[0]`` is
+        `rows_order` -- the stable order the sorted contraction runs -- followed by the
+        padding rows (mask 0, which a stable sort keeps after every real row)."""
         K = kmap.shape[1]
         per = (K + splits - 1) // splits
         valid = kmap >= 0
@@ -252,7 +266,12 @@ class SparseConvPackInfo:
                 k = s * per + i
                 if k < K:
                     bits[s] |= valid[:, k].to(torch.int32) << i
-        smask, order = torch.sort(bits, dim=-1, descending=True)
+        if splits == 1 and K <= 31:
+            pad = torch.arange(n_rows, kmap.shape[0], dtype=torch.long, device=kmap.device)
+            order = torch.cat([rows_order.long(), pad])[None]
+            smask = bits.gather(1, order)
+        else:
+            smask, order = torch.sort(bits, dim=-1, descending=True, stable=True)
         reorder_loc = order.to(torch.int32)
         reordered = kmap[order[0].long()] if splits == 1 else kmap[order.long()]
         return {"sorted_mask": smask, "reorder_loc": reorder_loc, "reorder_out_in_map": reordered,

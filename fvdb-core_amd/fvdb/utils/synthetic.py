@@ -7,7 +7,12 @@
   voxelised directly on a 2 cm lattice of a room of up to 10 x 10 x 3 m.
 
 Points are generated on the device with a seeded ``torch.Generator`` and voxelised by
-``GridBatch.from_points`` (``ijk = round(p / voxel_size)``).
+``GridBatch.from_points`` (``ijk = round((p - origin) / voxel_size)``) through ``voxelize``, whose
+origin puts every generated point at a non-negative voxel index: the sphere generators reach
+down to -0.3 (centres >= 0, radii <= 0.3), so with origin 0 a grid would straddle the 4096^3
+root tiles around 0.  ``voxelize`` keeps each workload grid inside the root tile [0, 4096)^3, as
+the reference's benchmark dataset shifts its grids to non-negative coordinates
+(``tests/benchmark/fvdb_benchmark/dataset.py:77-80``).
 """
 from __future__ import annotations
 
@@ -15,6 +20,42 @@ import math
 from typing import List, Sequence, Tuple
 
 import torch
+
+
+# Lowest coordinate of any point the generators below produce (sphere centres in [0, 1]^3,
+# radii up to 0.3; planes are clipped to the unit cube).
+POINTS_LO = -0.3
+ROOT_TILE = 4096
+
+
+def nonnegative_origin(voxel_size: float, lo: float = POINTS_LO) -> List[float]:
+    """Origin (world position of voxel (0, 0, 0)) at which every point with coordinates >= `lo`
+    rounds to a voxel index >= 1."""
+    o = float(lo) - float(voxel_size)
+    return [o, o, o]
+
+
+def voxelize(points, voxel_sizes, device=None):
+    """GridBatch of `points` (a tensor or a list of tensors, one per grid) at `voxel_sizes`
+    (one float, or one per grid) with ``nonnegative_origin`` origins.  Raises if a voxel
+    lands outside [0, 4096)^3."""
+    import fvdb
+
+    pts = points if isinstance(points, (list, tuple)) else [points]
+    vs = [float(v) for v in voxel_sizes] if isinstance(voxel_sizes, (list, tuple)) else [float(voxel_sizes)] * len(pts)
+    if len(vs) != len(pts):
+        raise ValueError("one voxel size per grid")
+    arg = fvdb.JaggedTensor(list(pts)) if isinstance(points, (list, tuple)) else points
+    grid = fvdb.GridBatch.from_points(arg, voxel_sizes=[[v, v, v] for v in vs],
+                                      origins=[nonnegative_origin(v) for v in vs], device=device)
+    check_root_tile(grid.ijk.jdata)
+    return grid
+
+
+def check_root_tile(ijk: torch.Tensor) -> None:
+    """Raises ValueError unless every coordinate lies in [0, 4096)."""
+    if ijk.numel() and (int(ijk.min()) < 0 or int(ijk.max()) >= ROOT_TILE):
+        raise ValueError(f"voxel coordinates [{int(ijk.min())}, {int(ijk.max())}] leave the root tile [0, 4096)")
 
 
 def _gen(seed: int, device) -> torch.Generator:

@@ -389,3 +389,78 @@ def row_order(kmap: np.ndarray, window: int = 0) -> np.ndarray:
     rows = np.arange(n, dtype=np.int64)
     win = rows // window if window > 0 else np.zeros(n, np.int64)
     return np.lexsort((rows, -mask, win)).astype(np.int32)
+
+
+# ---------------------------------------------------------------------------------
+# voxelisation of points, dense read/write, ijk -> index (the steps either side of the
+# convolution that produce and consume its rows)
+# ---------------------------------------------------------------------------------
+
+def points_to_ijk(points: np.ndarray, voxel_size, origin) -> Tuple[np.ndarray, np.ndarray]:
+    """Voxel coordinates of points and the distance of each pre-rounding coordinate to a
+    rounding tie.
+
+    ``ijkForPointsCallback`` (``src/fvdb/detail/ops/BuildGridFromPoints.cu:28-44``) applies the
+    primal transform of ``VoxelCoordTransform.h:296-307`` -- scale = 1 / voxel_size and
+    translate = -origin / voxel_size, both computed in float64 and used in the point
+    precision (``:88-107``) -- as ``p * scale + translate`` in the point precision, then
+    rounds with NanoVDB's ``Vec3::round`` (x -> floor(x + 0.5)).  The returned distances
+    (float64, per coordinate: |frac(v) - 0.5|) let a caller skip points that sit within an
+    ulp or two of a tie, where a fused multiply-add and two roundings may round apart."""
+    p = np.asarray(points)
+    dt = p.dtype.type
+    vs = np.asarray(voxel_size, dtype=np.float64).reshape(-1)
+    og = np.asarray(origin, dtype=np.float64).reshape(-1)
+    vs = np.broadcast_to(vs, (3,)) if vs.size == 1 else vs
+    og = np.broadcast_to(og, (3,)) if og.size == 1 else og
+    scale = (1.0 / vs).astype(dt)
+    trans = (-og / vs).astype(dt)
+    v = (p * scale).astype(dt) + trans
+    v = v.astype(dt)
+    ijk = np.floor(v.astype(np.float64) + 0.5).astype(np.int32)
+    frac = v.astype(np.float64) - np.floor(v.astype(np.float64))
+    return ijk, np.abs(frac - 0.5)
+
+
+def ijk_to_index(grids: Sequence[np.ndarray], queries: Sequence[np.ndarray]) -> np.ndarray:
+    """Cumulative index of each query voxel (per grid b, in tree order across grids), or -1
+    (``src/fvdb/detail/ops/IjkToIndex.cu``: ``-1`` for inactive voxels)."""
+    index = _index_of(grids)
+    out = []
+    for b, q in enumerate(queries):
+        for c in np.asarray(q).reshape(-1, 3).tolist():
+            out.append(index.get((b, c[0], c[1], c[2]), -1))
+    return np.asarray(out, dtype=np.int64)
+
+
+def write_to_dense(grids: Sequence[np.ndarray], data: np.ndarray, min_coord: np.ndarray, size) -> np.ndarray:
+    """Dense [B, X, Y, Z, C] array: row n of grid b lands at ijk - min_coord[b] when inside
+    [0, size); zeros elsewhere (``autograd/ReadIntoDense.cpp:20-80`` with
+    ``ops/ReadIntoDense.cu``)."""
+    B = len(grids)
+    C = data.shape[1]
+    out = np.zeros((B,) + tuple(int(s) for s in size) + (C,), dtype=data.dtype)
+    base = 0
+    for b, g in enumerate(grids):
+        g = np.asarray(g).reshape(-1, 3)
+        for n, c in enumerate(g.tolist()):
+            r = [c[d] - int(min_coord[b][d]) for d in range(3)]
+            if all(0 <= r[d] < size[d] for d in range(3)):
+                out[b, r[0], r[1], r[2]] = data[base + n]
+        base += len(g)
+    return out
+
+
+def read_from_dense(grids: Sequence[np.ndarray], dense: np.ndarray, origins: np.ndarray) -> np.ndarray:
+    """Rows [N, C] read at ijk - origins[b] from a dense [B, X, Y, Z, C] array, zeros
+    outside it (``autograd/ReadFromDense.cpp:18-60`` with ``ops/ReadFromDense.cu``)."""
+    rows = []
+    size = dense.shape[1:4]
+    for b, g in enumerate(grids):
+        for c in np.asarray(g).reshape(-1, 3).tolist():
+            r = [c[d] - int(origins[b][d]) for d in range(3)]
+            if all(0 <= r[d] < size[d] for d in range(3)):
+                rows.append(dense[b, r[0], r[1], r[2]])
+            else:
+                rows.append(np.zeros(dense.shape[4:], dtype=dense.dtype))
+    return np.stack(rows) if rows else np.zeros((0,) + dense.shape[4:], dtype=dense.dtype)

@@ -202,52 +202,92 @@ def dense_box_grid(fvdb, dims):
     return fvdb.GridBatch.from_dense(1, dims, device="cuda")
 
 
-@pytest.mark.parametrize("ks", [2, 3, 4, 5])
-@pytest.mark.parametrize("stride", [1, 2])
-@pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.bfloat16])
-def test_dense_conv3d_equivalence(ks, stride, dtype):
-    """``tests/unit/test_conv.py:361-461`` on a 12^3 dense grid (4 -> 8 channels)."""
+GRAD_TOL = {torch.float64: 1e-12, torch.float32: 1e-5, torch.float16: 4e-3, torch.bfloat16: 2e-2}
+
+
+def _build(pack, backend):
+    if backend == "gather_scatter":
+        pack.build_gather_scatter()
+        return "GATHER_SCATTER"
+    pack.build_implicit_gemm(sorted=backend == "igemm_sorted", training=True)
+    return "IGEMM"
+
+
+@pytest.mark.parametrize("backend", ["gather_scatter", "igemm", "igemm_sorted"])
+@pytest.mark.parametrize("ks", [2, 3, 4, 5, 6])
+@pytest.mark.parametrize("stride", [1, 2, 4])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.float16, torch.bfloat16])
+def test_dense_conv3d_equivalence(ks, stride, dtype, backend):
+    """``tests/unit/test_conv.py:361-461`` on a 32^3 dense grid (4 -> 8 channels): forward, dgrad and
+    wgrad against ``conv3d`` and its autograd in float64 on the CPU, fed the same rounded inputs.
+    Target voxels outside the dense output's range (the sparse target grid of a strided conv can
+    reach one coarse voxel past it) are left out, as the reference crops them."""
     fvdb = _fvdb()
-    print(f"[case] dense ks={ks} stride={stride} {dtype}", flush=True)
-    torch.manual_seed(0)
-    grid = dense_box_grid(fvdb, (12, 12, 12))
-    x = torch.randn(grid.total_voxels, 4, dtype=torch.float64)
+    print(f"[case] dense {backend} ks={ks} stride={stride} {dtype}", flush=True)
+    g = torch.Generator().manual_seed(ks * 10 + stride)
+    D = 32
+    grid = dense_box_grid(fvdb, (D, D, D))
+    x = torch.randn(grid.total_voxels, 4, generator=g, dtype=torch.float64).to(dtype)
     w = torch.empty(8, 4, ks, ks, ks, dtype=torch.float64)
-    torch.nn.init.kaiming_uniform_(w, a=math.sqrt(5))
+    torch.nn.init.kaiming_uniform_(w, a=math.sqrt(5), generator=g)
+    w = w.to(dtype)
     pack, tgt = grid.sparse_conv_kernel_map(ks, stride)
-    pack.build_gather_scatter()
-    out = pack.sparse_conv_3d(x.to("cuda", dtype), w.to("cuda", dtype), fvdb.ConvPackBackend.GATHER_SCATTER)
+    be = getattr(fvdb.ConvPackBackend, _build(pack, backend))
+    xd, wd = x.cuda().requires_grad_(True), w.cuda().requires_grad_(True)
+    out = pack.sparse_conv_3d(xd, wd, be).jdata
+    # dense reference: [C, z, y, x] volume, float64
     ijk = grid.ijk.jdata.long().cpu()
-    dense = torch.zeros(4, 12, 12, 12, dtype=torch.float64)
-    xr = x.to(dtype).double()
+    xr = x.double().requires_grad_(True)
+    wr = w.double().requires_grad_(True)
+    dense = torch.zeros(4, D, D, D, dtype=torch.float64)
     dense[:, ijk[:, 2], ijk[:, 1], ijk[:, 0]] = xr.t()
-    ref = torch.nn.functional.conv3d(dense[None], w.to(dtype).double(), padding=(ks - 1) // 2, stride=stride)[0]
+    ref = torch.nn.functional.conv3d(dense[None], wr, padding=(ks - 1) // 2, stride=stride)[0]
     t = tgt.ijk.jdata.long().cpu()
     keep = (t[:, 0] < ref.shape[3]) & (t[:, 1] < ref.shape[2]) & (t[:, 2] < ref.shape[1]) & (t >= 0).all(1)
     ref_rows = ref[:, t[keep, 2], t[keep, 1], t[keep, 0]].t()
-    err = rel_err(out.jdata.cpu()[keep], ref_rows)
-    print(f"  rel err {err:.2e}", flush=True)
-    assert err <= REL_TOL[dtype]
+    tol = REL_TOL[dtype]
+    e = rel_err(out.detach().cpu()[keep], ref_rows)
+    # gradients of a loss that weighs every kept output row
+    R = torch.randn(ref_rows.shape, generator=g, dtype=torch.float64).to(dtype).double()
+    (ref_rows * R).sum().backward()
+    Rf = torch.zeros(out.shape, dtype=dtype)
+    Rf[keep] = R.to(dtype)
+    (out * Rf.cuda()).sum().backward()
+    e_gi, e_gw = rel_err(xd.grad, xr.grad), rel_err(wd.grad, wr.grad)
+    print(f"  out {e:.2e} dgrad {e_gi:.2e} wgrad {e_gw:.2e}", flush=True)
+    assert e <= tol and e_gi <= GRAD_TOL[dtype] and e_gw <= GRAD_TOL[dtype]
 
 
-@pytest.mark.parametrize("ks", [2, 3, 4])
-def test_transposed_vs_conv_transpose3d(ks):
+@pytest.mark.parametrize("backend", ["gather_scatter", "igemm"])
+@pytest.mark.parametrize("ks", [2, 3, 4, 5, 6])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.float16, torch.bfloat16])
+def test_transposed_vs_conv_transpose3d(ks, dtype, backend):
+    """``tests/unit/test_conv.py:579-675``: transposed conv (stride 1) from the target grid back onto
+    a 16^3 dense source grid against ``conv_transpose3d`` in float64, forward and gradients."""
     fvdb = _fvdb()
-    torch.manual_seed(1)
-    grid = dense_box_grid(fvdb, (10, 10, 10))
+    print(f"[case] transposed {backend} ks={ks} {dtype}", flush=True)
+    g = torch.Generator().manual_seed(ks)
+    D = 16
+    grid = dense_box_grid(fvdb, (D, D, D))
     pack, tgt = grid.sparse_conv_kernel_map(ks, 1)
-    pack.build_implicit_gemm(training=True)
-    x = torch.randn(tgt.total_voxels, 4, dtype=torch.float64)
-    w = torch.randn(4, 6, ks, ks, ks, dtype=torch.float64)
-    out = pack.sparse_transpose_conv_3d(x.float().cuda(), w.float().cuda(), fvdb.ConvPackBackend.IGEMM)
+    be = getattr(fvdb.ConvPackBackend, _build(pack, backend))
+    x = torch.randn(tgt.total_voxels, 4, generator=g, dtype=torch.float64).to(dtype)
+    w = (torch.randn(4, 6, ks, ks, ks, generator=g, dtype=torch.float64) / math.sqrt(4 * ks ** 3)).to(dtype)
+    xd, wd = x.cuda().requires_grad_(True), w.cuda().requires_grad_(True)
+    out = pack.sparse_transpose_conv_3d(xd, wd, be).jdata
     ijk = tgt.ijk.jdata.long().cpu()
-    dense = torch.zeros(4, 10, 10, 10, dtype=torch.float64)
-    dense[:, ijk[:, 2], ijk[:, 1], ijk[:, 0]] = x.float().double().t()
-    ref = torch.nn.functional.conv_transpose3d(dense[None], w.float().double(), padding=(ks - 1) // 2)[0]
-    ref = ref[:, :10, :10, :10]
+    xr, wr = x.double().requires_grad_(True), w.double().requires_grad_(True)
+    dense = torch.zeros(4, D, D, D, dtype=torch.float64)
+    dense[:, ijk[:, 2], ijk[:, 1], ijk[:, 0]] = xr.t()
+    ref = torch.nn.functional.conv_transpose3d(dense[None], wr, padding=(ks - 1) // 2)[0][:, :D, :D, :D]
     s = grid.ijk.jdata.long().cpu()
     ref_rows = ref[:, s[:, 2], s[:, 1], s[:, 0]].t()
-    assert rel_err(out.jdata.cpu(), ref_rows) <= 1e-5
+    R = torch.randn(ref_rows.shape, generator=g, dtype=torch.float64).to(dtype)
+    (ref_rows * R.double()).sum().backward()
+    (out * R.cuda()).sum().backward()
+    e, e_gi, e_gw = rel_err(out, ref_rows), rel_err(xd.grad, xr.grad), rel_err(wd.grad, wr.grad)
+    print(f"  out {e:.2e} dgrad {e_gi:.2e} wgrad {e_gw:.2e}", flush=True)
+    assert e <= REL_TOL[dtype] and e_gi <= GRAD_TOL[dtype] and e_gw <= GRAD_TOL[dtype]
 
 
 def test_error_inputs_raise_value_error():
@@ -412,10 +452,10 @@ def test_direct_lds_tile_bit_exact(dtype, cout, large, ks, st):
     2x2x2 layers with 128k outputs there take 128-channel workgroups."""
     fvdb = _fvdb()
     from fvdb._lib import ext
-    from fvdb.utils.synthetic import sphere_points
+    from fvdb.utils.synthetic import sphere_points, voxelize
 
     n_points, vs = (1_500_000, 1 / 512) if large else (60_000, 1 / 96)
-    grid = fvdb.GridBatch.from_points(sphere_points(seed=5, n_points=n_points, device="cuda"), voxel_sizes=vs)
+    grid = voxelize(sphere_points(seed=5, n_points=n_points, device="cuda"), vs)
     pack, tgt = grid.sparse_conv_kernel_map(ks, st)
     n_in, n = grid.total_voxels, tgt.total_voxels
     kmap = pack._map_fwd()
@@ -449,9 +489,9 @@ def test_direct_lds_wide_bit_exact(dtype, cout):
     (transposed map)."""
     fvdb = _fvdb()
     from fvdb._lib import ext
-    from fvdb.utils.synthetic import sphere_points
+    from fvdb.utils.synthetic import sphere_points, voxelize
 
-    grid = fvdb.GridBatch.from_points(sphere_points(seed=5, n_points=1_500_000, device="cuda"), voxel_sizes=1 / 512)
+    grid = voxelize(sphere_points(seed=5, n_points=1_500_000, device="cuda"), 1 / 512)
     pack, _ = grid.sparse_conv_kernel_map(3, 1)
     pack.build_implicit_gemm(training=True)
     n = grid.total_voxels
@@ -475,11 +515,11 @@ def test_row_orders_match_oracle():
     tap mask, ties in row order, windows in place), on one block and on many, and the
     contraction through them equals the unordered one bit for bit."""
     fvdb = _fvdb()
-    from fvdb.utils.synthetic import sphere_points
+    from fvdb.utils.synthetic import sphere_points, voxelize
 
     z = golden("sphere_k3s1_c8.npz")
     small = grids_from_fixture(fvdb, z)
-    big = fvdb.GridBatch.from_points(sphere_points(seed=4, n_points=200_000, device="cuda"), voxel_sizes=1 / 160)
+    big = voxelize(sphere_points(seed=4, n_points=200_000, device="cuda"), 1 / 160)
     g = torch.Generator().manual_seed(3)
     for grid in (small, big):
         for ks, st in ((3, 1), (2, 2)):
@@ -508,9 +548,9 @@ def test_automatic_row_order_policy():
     (fp32; 16-bit at 128+ input channels), and the results do not depend on it:
     bit-identical to tree order, forward and backward."""
     fvdb = _fvdb()
-    from fvdb.utils.synthetic import sphere_points
+    from fvdb.utils.synthetic import sphere_points, voxelize
 
-    grid = fvdb.GridBatch.from_points(sphere_points(seed=2, n_points=100_000, device="cuda"), voxel_sizes=1 / 128)
+    grid = voxelize(sphere_points(seed=2, n_points=100_000, device="cuda"), 1 / 128)
     p3, _ = grid.sparse_conv_kernel_map(3, 1)
     p2, coarse = grid.sparse_conv_kernel_map(2, 2)
     p3.build_implicit_gemm(training=True)

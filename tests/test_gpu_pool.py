@@ -172,11 +172,11 @@ def test_coarsen_from_leaves_bit_exact(factor):
     grids, and one of 700k voxels on sphere shells."""
     fvdb = _fvdb()
     from fvdb._lib import ext
-    from fvdb.utils.synthetic import sphere_points
+    from fvdb.utils.synthetic import sphere_points, voxelize
 
     rng = np.random.default_rng(sum(factor))
     small, small_np = grids(fvdb, rng, [900, 1, 400], 60)
-    big = fvdb.GridBatch.from_points(sphere_points(seed=9, n_points=700_000, device="cuda"), voxel_sizes=1 / 300)
+    big = voxelize(sphere_points(seed=9, n_points=700_000, device="cuda"), 1 / 300)
     for g, per in ((small, small_np), (big, None)):
         print(f"[case] coarsen factor={factor} voxels={g.total_voxels}", flush=True)
         fast = g.coarsened_grid(factor)

@@ -300,3 +300,74 @@ def test_golden_fixture(fname):
     w = torch.from_numpy(z["weight"])
     out = oracle.sparse_conv(x, w, km, km.shape[0], st)
     assert torch.allclose(out, torch.from_numpy(z["out"]), atol=1e-12, rtol=1e-12)
+
+
+# --------------------------------------------------------------------------------------
+# voxelisation of points, ijk -> index, dense read/write
+# --------------------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_points_to_ijk_closed_form(dtype):
+    """Points at voxel centres and at +-0.3 voxel from them map to that voxel; voxel sizes are
+    powers of two so every value is exact.  A sign slip on the origin, a multiplied instead of
+    divided voxel size or a floor instead of a round fails here."""
+    rng = np.random.default_rng(0)
+    for vs, og in ((0.25, (0.0, 0.0, 0.0)), (0.125, (-1.5, 2.0, 0.375)), (2.0, (3.0, -4.0, 10.0))):
+        i = rng.integers(-40, 40, size=(200, 3))
+        for d in (-0.3, 0.0, 0.3):
+            p = (np.asarray(og) + (i + d) * vs).astype(dtype)
+            ijk, tie = oracle.points_to_ijk(p, vs, og)
+            assert np.array_equal(ijk, i), (vs, og, d)
+            assert (tie >= 0.19).all()
+    # exactly half-way rounds up (NanoVDB's floor(x + 0.5)); flagged as a tie
+    ijk, tie = oracle.points_to_ijk(np.array([[0.5, -0.5, 1.5]], dtype), 1.0, 0.0)
+    assert ijk.tolist() == [[1, 0, 2]] and (tie == 0).all()
+
+
+def test_points_to_ijk_vs_exact_rationals():
+    """float64 points away from ties: the same voxel as exact rational arithmetic gives."""
+    from fractions import Fraction
+
+    rng = np.random.default_rng(1)
+    p = rng.uniform(-3, 3, size=(300, 3))
+    vs, og = 1 / 96, (0.2, -0.7, 1.1)
+    ijk, tie = oracle.points_to_ijk(p, vs, og)
+    keep = (tie > 1e-6).all(1)
+    assert keep.sum() > 280
+    for row, q in zip(ijk[keep], p[keep]):
+        exact = [math.floor((Fraction(float(q[d])) - Fraction(og[d])) / Fraction(vs) + Fraction(1, 2)) for d in range(3)]
+        assert row.tolist() == exact
+
+
+def test_ijk_to_index_invariants():
+    rng = np.random.default_rng(2)
+    grids = [oracle.sort_unique(random_voxels(rng, 300, 12)), oracle.sort_unique(random_voxels(rng, 200, 9, lo=-4))]
+    n0 = len(grids[0])
+    # every voxel finds itself (cumulative index across grids)
+    assert np.array_equal(oracle.ijk_to_index(grids, grids), np.arange(n0 + len(grids[1])))
+    # coordinates outside a grid's set are -1; grid 1's voxels are not looked up in grid 0
+    far = [np.array([[100, 100, 100]], np.int32), grids[0][:5]]
+    got = oracle.ijk_to_index(grids, far)
+    in1 = {tuple(c) for c in grids[1].tolist()}
+    exp = [-1] + [n0 + int(np.flatnonzero((grids[1] == c).all(1))[0]) if tuple(c) in in1 else -1
+                  for c in grids[0][:5].tolist()]
+    assert got.tolist() == exp
+
+
+def test_dense_write_read_closed_form():
+    """A dense box written at its own minimum is a reshape of the rows (x, y, z order); reading
+    it back is the identity; a shifted frame crops and pads with zeros."""
+    box = dense_box((4, 3, 5), lo=(2, -1, 7))
+    g = oracle.sort_unique(box)
+    rng = np.random.default_rng(3)
+    data = rng.standard_normal((len(g), 2))
+    lo = np.array([[2, -1, 7]])
+    dense = oracle.write_to_dense([g], data, lo, (4, 3, 5))
+    for n, c in enumerate(g.tolist()):
+        assert np.array_equal(dense[0, c[0] - 2, c[1] + 1, c[2] - 7], data[n])
+    assert np.array_equal(oracle.read_from_dense([g], dense, lo), data)
+    shifted = oracle.write_to_dense([g], data, np.array([[3, -1, 7]]), (4, 3, 5))
+    assert np.array_equal(shifted[0, :3], dense[0, 1:]) and not shifted[0, 3].any()
+    back = oracle.read_from_dense([g], shifted, np.array([[3, -1, 7]]))
+    assert np.array_equal(back[g[:, 0] >= 3], data[g[:, 0] >= 3]) and not back[g[:, 0] < 3].any()

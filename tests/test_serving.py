@@ -109,28 +109,57 @@ def test_latency_stats():
     assert math.isclose(s["p50_ms"], 2.0) and math.isclose(s["p99_ms"], 4.0)
 
 
+def test_exact_bytes_lowers_reservation_after_ingest():
+    """An executor that knows a request's footprint after ingest gives back the slack of the
+    admission bound, so a request that waited on the bound is admitted once the first is ingested."""
+
+    class Exact(FakeExecutor):
+        def exact_bytes(self, r):
+            return 2 * r.ijk.shape[0]
+
+    ex = Exact(bytes_per_voxel=10, done_after=50)
+    b = ContinuousBatcher(ex, hbm_budget_bytes=1500, max_batch_voxels=10, max_inflight=1)
+    for r in reqs([100, 100]):
+        b.submit(r)
+    b.step()
+    # first admitted at 1000, lowered to 200 after ingest; the second (1000) fits beside it
+    assert b.reserved == 400 and len(b.prepared) + len(b.inflight) == 2
+    out = b.serve([])
+    assert all(r.status == "done" for r in out) and b.reserved == 0
+
+
+def _requests(fvdb, sizes, seed0):
+    from fvdb.utils.synthetic import sphere_points_for_voxels, voxelize
+
+    rs = []
+    for i, nv in enumerate(sizes):
+        pts, vs = sphere_points_for_voxels(seed0 + i, nv, device="cuda")
+        g = voxelize(pts, vs)
+        rs.append(Request(rid=i, ijk=g.ijk.jdata.cpu(), seed=100 + i))
+    return rs
+
+
 @pytest.mark.gpu
 def test_gpu_engine_matches_module_alone_and_in_mix():
+    """The serving workload's network (4 levels, 32/64/128/256) on requests of 20k to ~1.2M
+    voxels: each result is bit-identical alone and in the mix, and equal to the fvdb.nn module's."""
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
     import fvdb
     from fvdb.serving import UNetExecutor
-    from fvdb.utils.synthetic import sphere_points
 
     torch.manual_seed(0)
-    model = fvdb.nn.SparseUNet(in_channels=32, channels=(32, 64, 128)).cuda().eval()
-    rs = []
-    for i, (n, vs) in enumerate([(40_000, 1 / 64), (120_000, 1 / 128), (20_000, 1 / 48)]):
-        pts = sphere_points(seed=10 + i, n_points=n, device="cuda")
-        g = fvdb.GridBatch.from_points(pts, voxel_sizes=vs)
-        rs.append(Request(rid=i, ijk=g.ijk.jdata.cpu(), seed=100 + i))
+    model = fvdb.nn.SparseUNet(in_channels=32, channels=(32, 64, 128, 256)).cuda().eval()
+    rs = _requests(fvdb, [40_000, 1_200_000, 20_000], 10)
+    assert rs[1].ijk.shape[0] >= 1_000_000
+    print(f"[serve] request voxels {[r.ijk.shape[0] for r in rs]}", flush=True)
     for dtype in (torch.bfloat16, torch.float32):
         ex = UNetExecutor(model, dtype=dtype)
         alone = []
         for r in rs:
-            b = ContinuousBatcher(ex, hbm_budget_bytes=8 << 30)
+            b = ContinuousBatcher(ex, hbm_budget_bytes=64 << 30)
             alone.append(b.serve([Request(rid=r.rid, ijk=r.ijk, seed=r.seed)])[0])
-        b = ContinuousBatcher(ex, hbm_budget_bytes=8 << 30, max_batch_voxels=10**8)
+        b = ContinuousBatcher(ex, hbm_budget_bytes=64 << 30, max_batch_voxels=10**8)
         mixed = b.serve([Request(rid=r.rid, ijk=r.ijk, seed=r.seed) for r in rs])
         assert b.batches[0] == 3, b.batches
         for a, m in zip(alone, mixed):
@@ -146,3 +175,40 @@ def test_gpu_engine_matches_module_alone_and_in_mix():
                 y = mdl(fvdb.nn.VDBTensor(grid, grid.jagged_like(x)))
                 assert torch.equal(y.data.jdata, a.result)
         model = model.float()
+
+
+@pytest.mark.gpu
+def test_gpu_admission_bounds_measured_memory():
+    """A stream under a budget of 1.2 times the largest request's admission bound: the budget
+    binds (requests wait for memory), every request completes, and the device memory the stream
+    allocates beyond what was live before it never exceeds the budget."""
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import fvdb
+    from fvdb.serving import UNetExecutor
+
+    torch.manual_seed(0)
+    model = fvdb.nn.SparseUNet(in_channels=32, channels=(32, 64, 128, 256)).cuda().eval()
+    rs = _requests(fvdb, [300_000, 900_000, 60_000, 600_000, 150_000, 800_000, 30_000, 400_000], 40)
+    ex = UNetExecutor(model, dtype=torch.bfloat16, host_results=True)
+    budget = int(1.2 * max(ex.estimate_bytes(r) for r in rs))
+    torch.cuda.synchronize()
+    base = torch.cuda.memory_allocated()
+    torch.cuda.reset_peak_memory_stats()
+    b = ContinuousBatcher(ex, hbm_budget_bytes=budget, max_batch_voxels=4_000_000, max_inflight=2)
+    waited = []
+    orig = b._admit
+
+    def admit():
+        orig()
+        waited.append(len(b.queue))
+    b._admit = admit
+    out = b.serve([Request(rid=r.rid, ijk=r.ijk, seed=r.seed) for r in rs])
+    torch.cuda.synchronize()
+    peak = torch.cuda.max_memory_allocated() - base
+    print(f"[serve budget] budget {budget / 1e9:.2f} GB, peak reserved by admission {b.peak_reserved / 1e9:.2f} GB, "
+          f"measured peak {peak / 1e9:.2f} GB, batches {b.batches}", flush=True)
+    assert all(r.status == "done" for r in out)
+    assert max(waited) > 0, "the budget never held a request back"
+    assert b.peak_reserved <= budget and b.reserved == 0
+    assert peak <= budget
