@@ -3,6 +3,8 @@
 // scalar fp64 form, and the split-K weight gradient with a compensated reduction.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "fvdbk_common.h"
 #include "fvdbk_conv_frag.h"
 #include "fvdbk_conv_tile.h"
@@ -477,6 +479,41 @@ __global__ void k_row_keys(const int32_t* __restrict__ map, int64_t n_rows, int 
 
 }  // namespace fvdbk
 
+// ---------------------------------------------------------------------------------
+// library-owned scratch of the no-allocation weight gradient
+//
+// File-scope device array: every GPU that loads the library holds its own copy for the
+// life of the process.  Host code never passes the array's name to a kernel (on the host
+// it is only a stand-in for the symbol); it uses the device address that
+// hipGetSymbolAddress returns, looked up once per device with that device current.
+// ---------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[FVDBK_WGRAD_SCRATCH_BYTES];
+
+namespace fvdbk {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array (looked up on its first use).
+int scratch_address(void** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+        return set_error(FVDBK_EHIP, "no current HIP device for the wgrad scratch");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(FVDBK_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return FVDBK_OK;
+}
+}  // namespace
+}  // namespace fvdbk
+
 using namespace fvdbk;
 
 extern "C" {
@@ -561,6 +598,24 @@ int fvdbk_conv_wgrad(fvdbk_dtype dtype, const void* in, int32_t c_in, const void
     default:
         return set_error(FVDBK_ETYPE, "unsupported dtype %d", (int)dtype);
     }
+}
+
+size_t fvdbk_conv_wgrad_scratch_bytes(void) { return (size_t)FVDBK_WGRAD_SCRATCH_BYTES; }
+
+int fvdbk_conv_wgrad_noalloc(fvdbk_dtype dtype, const void* in, int32_t c_in, const void* gout, int32_t c_out,
+                             const int32_t* pairs, const int64_t* k_offsets, const int32_t* chunk_start, int32_t K,
+                             int32_t chunk, int64_t total_chunks, int32_t transposed, void* gw, hipStream_t stream) {
+    const size_t need = fvdbk_conv_wgrad_workspace_size(dtype, total_chunks, c_in, c_out);
+    FVDBK_REQUIRE(need <= (size_t)FVDBK_WGRAD_SCRATCH_BYTES, FVDBK_ENOSPC,
+                  "wgrad partials need %zu bytes (%lld chunks x %d x %d), over the %zu-byte scratch of "
+                  "fvdbk_conv_wgrad_noalloc: use larger chunks or fvdbk_conv_wgrad",
+                  need, (long long)total_chunks, (int)c_in, (int)c_out, (size_t)FVDBK_WGRAD_SCRATCH_BYTES);
+    void* ws = nullptr;
+    const int rc = scratch_address(&ws);
+    if (rc != FVDBK_OK) return rc;
+    // every chunk writes its whole partial before the reduction reads it: nothing to clear
+    return fvdbk_conv_wgrad(dtype, in, c_in, gout, c_out, pairs, k_offsets, chunk_start, K, chunk, total_chunks,
+                            transposed, ws, (size_t)FVDBK_WGRAD_SCRATCH_BYTES, gw, stream);
 }
 
 }  // extern "C"

@@ -314,6 +314,23 @@ int fvdbk_conv_wgrad(fvdbk_dtype dtype, const void* in, int32_t c_in, const void
                      int32_t chunk, int64_t total_chunks, int32_t transposed, void* workspace,
                      size_t workspace_bytes, void* gw, hipStream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * Weight gradient that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and result as fvdbk_conv_wgrad
+ * without the workspace: the split-K partials live in a device array owned by the library,
+ * FVDBK_WGRAD_SCRATCH_BYTES per GPU, held from the library's load until the process ends.
+ * Calls whose partials (fvdbk_conv_wgrad_workspace_size) exceed it fail with FVDBK_ENOSPC
+ * before any launch.  The entry point calls no hipMalloc/hipFree.  Every call on a GPU shares
+ * that GPU's array, so calls that overlap on one GPU (different streams or threads) race;
+ * the array keeps the last call's contents, and no call depends on them.
+ * ------------------------------------------------------------------------- */
+#define FVDBK_WGRAD_SCRATCH_BYTES (64u << 20)
+size_t fvdbk_conv_wgrad_scratch_bytes(void);
+int fvdbk_conv_wgrad_noalloc(fvdbk_dtype dtype, const void* in, int32_t c_in, const void* gout, int32_t c_out,
+                             const int32_t* pairs, const int64_t* k_offsets, const int32_t* chunk_start, int32_t K,
+                             int32_t chunk, int64_t total_chunks, int32_t transposed, void* gw, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
