@@ -211,6 +211,8 @@ def main() -> int:
                          "workloads, one JSON line each; orders: contraction per layer in tree order vs "
                          "tap-mask order per window")
     ap.add_argument("--requests", type=int, default=120, help="serve: number of requests")
+    ap.add_argument("--serve-requests", type=int, default=40,
+                    help="requests of the short serving stream reported in the default line (0: none)")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: Poisson arrival rate (req/s); 0 = auto")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -320,6 +322,20 @@ def main() -> int:
 
     errors, sample_n = ({}, 0) if args.no_check else check_errors(torch, fvdb, grid, x, wa, wb, 0)
     cpu = None if args.no_cpu_baseline else cpu_baseline(torch, fvdb, grid, x, wa, wb)
+    serve = None
+    if args.serve_requests > 0:
+        # the metric's serving half: a short Poisson stream through the U-Net workload's network
+        log(f"[bench] serving stream of {args.serve_requests} requests")
+        sv = serve_stream(torch, fvdb, dev, args.serve_requests)
+        cb = sv["bf16_continuous_batching"]
+        serve = {"workload": f"{args.serve_requests} single-grid requests (log-uniform 10k-2M voxels, Poisson "
+                             f"arrivals at 70% of the closed-batch service rate), 4-level U-Net 32/64/128/256",
+                 "p50_ms": cb["p50_ms"], "p99_ms": cb["p99_ms"], "throughput_voxels_per_s": cb["throughput_voxels_per_s"],
+                 "rate_req_per_s": cb["rate_req_per_s"], "mean_batch": cb["mean_batch"],
+                 "hbm_budget_GB": cb["hbm_budget_GB"], "measured_peak_allocated_GB": cb["measured_peak_allocated_GB"],
+                 "admission_steps_with_requests_waiting": cb["admission_steps_with_requests_waiting"],
+                 "bf16_one_at_a_time_p50_p99_ms": [sv["bf16_one_at_a_time"]["p50_ms"], sv["bf16_one_at_a_time"]["p99_ms"]],
+                 "fp32_one_at_a_time_p50_p99_ms": [sv["fp32_one_at_a_time"]["p50_ms"], sv["fp32_one_at_a_time"]["p99_ms"]]}
 
     value = N / t_bf16
     rec = {
@@ -339,7 +355,8 @@ def main() -> int:
                                "then 2x2x2 s2 64->128, kernel maps + target grid rebuilt every step "
                                "(target grid and map B built on a second stream while conv A runs)",
                    "grids": grid.grid_count, "voxels": N, "voxels_per_grid": counts, "voxels_convB_out": N_b,
-                   "pairs_convA": P_a, "pairs_convB": P_b, "voxel_sizes": vs},
+                   "pairs_convA": P_a, "pairs_convB": P_b, "voxel_sizes": vs,
+                   "coordinates": "every voxel in the root tile [0, 4096)^3 (origin -0.3 - voxel size)"},
         "tflops_step_bf16": flops / t_bf16 / 1e12,
         "step_ms_single_stream": t_bf16_serial * 1e3,
         "fp32_step_voxels_per_s": N / t_fp32,
@@ -354,6 +371,7 @@ def main() -> int:
         "error_sample_voxels": sample_n,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serve": serve,
     }
     print(json.dumps(rec), flush=True)
     return 0
@@ -457,7 +475,7 @@ def run_diff(torch, fvdb, dev, args):
         of, ob = pack._order_for(True, 128, dt, 128), pack._order_for(False, 128, dt, 128)
         fwd = lambda: ext.implicit_gemm(x, a_f, pack._map_fwd(), N, of)  # noqa: E731
         dgr = lambda: ext.implicit_gemm(R, kio, pack._map_bwd(), N, ob)  # noqa: E731
-        chunk = wgrad_chunk(P, dt)
+        chunk = wgrad_chunk(P, dt, 128, 128)
         wgr = lambda: ext.wgrad(x, R, pairs, sizes, False, chunk)  # noqa: E731
         t = {"fwd_ms": _median_ms(torch, fwd, 10), "dgrad_ms": _median_ms(torch, dgr, 10),
              "wgrad_ms": _median_ms(torch, wgr, 10)}
@@ -631,14 +649,19 @@ def run_unet(torch, fvdb, dev, args):
                                "fp32": rel(outs["fp32"], outs["fp64"])}}
 
 
-def run_serve(torch, fvdb, dev, args):
-    """Poisson stream of single grids (log-uniform 10k-2M voxels), the U-Net in bf16 with continuous
-    batching, against a one-at-a-time fp32 loop over the same stream."""
+def serve_stream(torch, fvdb, dev, nreq, rate=0.0, seed=42, budget_factor=1.2):
+    """Poisson stream of single grids (log-uniform 10k-2M voxels) through the 4-level U-Net in bf16
+    with continuous batching, against one-at-a-time loops (bf16 and fp32) over the same arrivals.
+
+    The HBM budget is `budget_factor` times the largest request's admission bound, so admission
+    binds; the device memory the stream allocates beyond what was live before it is measured
+    (peak allocated after ``reset_peak_memory_stats``) and reported beside the budget.  Results
+    are copied to host memory as they complete.  `rate` 0 offers 70% of the engine's service
+    rate measured on a closed batch of the same requests."""
     from fvdb.serving import ContinuousBatcher, Request, UNetExecutor, latency_stats
     from fvdb.utils.synthetic import log_uniform, sphere_points_for_voxels, voxelize
 
-    nreq = args.requests
-    sizes = [int(v) for v in log_uniform(42, 10_000, 2_000_000, nreq)]
+    sizes = [int(v) for v in log_uniform(seed, 10_000, 2_000_000, nreq)]
     reqs_ijk = []
     for i, nv in enumerate(sizes):
         pts, vs = sphere_points_for_voxels(1000 + i, nv, device=dev)
@@ -649,7 +672,6 @@ def run_serve(torch, fvdb, dev, args):
     torch.manual_seed(0)
     model = fvdb.nn.SparseUNet(in_channels=32, channels=(32, 64, 128, 256)).to(dev).eval()
 
-    # one-at-a-time fp32 loop (the baseline): requests in arrival order, each alone
     def one_at_a_time(arrivals, dtype):
         m = model.to(dtype)
         t0 = time.perf_counter()
@@ -661,47 +683,74 @@ def run_serve(torch, fvdb, dev, args):
                 grid = fvdb.GridBatch.from_ijk(reqs_ijk[i].to(dev, non_blocking=True))
                 gen = torch.Generator(device=dev).manual_seed(100 + i)
                 xx = torch.randn((grid.total_voxels, 32), generator=gen, device=dev).to(dtype)
-                m(fvdb.nn.VDBTensor(grid, grid.jagged_like(xx)))
-                torch.cuda.synchronize()
+                y = m(fvdb.nn.VDBTensor(grid, grid.jagged_like(xx))).data.jdata
+                torch.empty(y.shape, dtype=y.dtype, pin_memory=True).copy_(y)  # result to host, as the server
                 lat.append(time.perf_counter() - t0 - tarr)
         model.float()
         lat.sort()
         span = time.perf_counter() - t0
-        return {"p50_ms": 1e3 * lat[len(lat) // 2], "p99_ms": 1e3 * lat[min(len(lat) - 1, int(0.99 * len(lat)))],
+        k99 = min(len(lat) - 1, max(0, math.ceil(0.99 * len(lat)) - 1))
+        return {"p50_ms": 1e3 * lat[(len(lat) - 1) // 2], "p99_ms": 1e3 * lat[k99],
                 "throughput_voxels_per_s": sum(counts) / span, "throughput_req_per_s": nreq / span, "span_s": span}
 
-    # offered load: measure the bf16 engine's service rate on a closed batch, then run at ~70% of it
-    ex = UNetExecutor(model, dtype=torch.bfloat16)
-    warm = ContinuousBatcher(ex, hbm_budget_bytes=96 << 30)
+    ex = UNetExecutor(model, dtype=torch.bfloat16, host_results=True)
+    budget = int(budget_factor * max(ex.estimate_bytes(Request(rid=i, ijk=t)) for i, t in enumerate(reqs_ijk)))
+    warm = ContinuousBatcher(ex, hbm_budget_bytes=budget)
     warm.serve([Request(rid=i, ijk=reqs_ijk[i], seed=100 + i) for i in range(min(8, nreq))])
-    # as in the step bench: the set-up's objects out of the cyclic collector's full scans
+    # the set-up's objects out of the cyclic collector's full scans
     gc.collect()
     gc.freeze()
     t0 = time.perf_counter()
-    closed = ContinuousBatcher(ex, hbm_budget_bytes=96 << 30)
+    closed = ContinuousBatcher(ex, hbm_budget_bytes=budget)
     closed.serve([Request(rid=i, ijk=reqs_ijk[i], seed=100 + i) for i in range(nreq)])
     torch.cuda.synchronize()
     service_rate = nreq / (time.perf_counter() - t0)
-    rate = args.rate if args.rate > 0 else 0.7 * service_rate
+    rate = rate if rate > 0 else 0.7 * service_rate
     gexp = torch.Generator().manual_seed(7)
     gaps = torch.empty(nreq, dtype=torch.float64).exponential_(rate, generator=gexp)
     arrivals = torch.cumsum(gaps, 0).tolist()
-    batcher = ContinuousBatcher(ex, hbm_budget_bytes=96 << 30, max_batch_voxels=6_000_000, max_inflight=2)
+    batcher = ContinuousBatcher(ex, hbm_budget_bytes=budget, max_batch_voxels=6_000_000, max_inflight=2)
+    waits = [0]
+    admit = batcher._admit
+
+    def counted_admit():
+        admit()
+        waits[0] += 1 if batcher.queue else 0
+    batcher._admit = counted_admit
+    torch.cuda.synchronize()
+    base = torch.cuda.memory_allocated(dev)
+    torch.cuda.reset_peak_memory_stats(dev)
     t_start = time.perf_counter()
     done = batcher.serve([Request(rid=i, ijk=reqs_ijk[i], seed=100 + i, arrival=arrivals[i]) for i in range(nreq)])
     torch.cuda.synchronize()
     span = time.perf_counter() - t_start
+    peak = torch.cuda.max_memory_allocated(dev) - base
     ok = [r for r in done if r.status == "done"]
     stats = latency_stats(done)
+    slow = sorted(ok, key=lambda r: -r.latency)[:3]
+    log("[bench] slowest requests: " + ", ".join(f"#{r.rid} {counts[r.rid]} voxels arrived {r.arrival * 1e3:.1f} ms "
+                                                 f"admitted +{(r.t_admit - r.arrival) * 1e3:.1f} latency "
+                                                 f"{r.latency * 1e3:.1f} ms" for r in slow))
     serve = {"requests": nreq, "completed": len(ok), "rate_req_per_s": rate, "service_rate_closed": service_rate,
              **stats, "throughput_voxels_per_s": sum(counts[r.rid] for r in ok) / span,
              "throughput_req_per_s": len(ok) / span, "span_s": span,
              "mean_batch": statistics.mean(batcher.batches), "max_batch": max(batcher.batches),
-             "peak_reserved_GB": batcher.peak_reserved / 1e9,
-             "hbm_high_water_GB": torch.cuda.max_memory_allocated() / 1e9}
-    base = one_at_a_time(arrivals, torch.float32)
-    return {"workload": "serve: Poisson stream, single grids log-uniform 10k-2M voxels, 4-level U-Net",
-            "voxels_total": sum(counts), "bf16_continuous_batching": serve, "fp32_one_at_a_time": base}
+             "hbm_budget_GB": budget / 1e9, "peak_reserved_by_admission_GB": batcher.peak_reserved / 1e9,
+             "measured_peak_allocated_GB": peak / 1e9, "measured_peak_within_budget": bool(peak <= budget),
+             "admission_steps_with_requests_waiting": waits[0]}
+    log(f"[bench] serve bf16 continuous batching: {serve}")
+    base_bf16 = one_at_a_time(arrivals, torch.bfloat16)
+    log(f"[bench] serve bf16 one at a time: {base_bf16}")
+    base_fp32 = one_at_a_time(arrivals, torch.float32)
+    log(f"[bench] serve fp32 one at a time: {base_fp32}")
+    return {"voxels_total": sum(counts), "voxels_min": min(counts), "voxels_max": max(counts),
+            "bf16_continuous_batching": serve, "bf16_one_at_a_time": base_bf16, "fp32_one_at_a_time": base_fp32}
+
+
+def run_serve(torch, fvdb, dev, args):
+    """The serving workload: a Poisson stream of `--requests` single grids (default 120)."""
+    rec = serve_stream(torch, fvdb, dev, args.requests, args.rate)
+    return {"workload": "serve: Poisson stream, single grids log-uniform 10k-2M voxels, 4-level U-Net", **rec}
 
 
 if __name__ == "__main__":

@@ -41,15 +41,34 @@ def kio_to_weight(gk: torch.Tensor, kernel_size, transposed: bool) -> torch.Tens
     return g.permute(4, 3, 2, 1, 0) if not transposed else g.permute(3, 4, 2, 1, 0)
 
 
-def wgrad_chunk(n_pairs: int, dtype: torch.dtype = torch.float32) -> int:
-    """Pairs per split-K chunk, a multiple of 64 in [256, 16384].  fp32 (MFMA-bound) runs
-    best with ~4096 chunks per launch; 16-bit inputs with ~768, since their contraction is
-    fast and every chunk writes a [Cin, Cout] fp32 partial that the reduction reads back
-    (measured on MI355X: 6.7M pairs, 128 channels, bf16 0.94 -> 0.73 ms)."""
-    target = 768 if dtype in (torch.bfloat16, torch.float16) else 4096
+def _wgrad_tile(c: int) -> int:
+    """Channels per workgroup tile along one side of the wgrad kernel (32, 64 or 128)."""
+    return 32 if c <= 32 else (64 if c <= 64 else 128)
+
+
+def wgrad_chunk(n_pairs: int, dtype: torch.dtype = torch.float32, cin: int = 128, cout: int = 128,
+                n_cu: int = 0) -> int:
+    """Pairs per split-K chunk, a multiple of 64 and at least 256.
+
+    fp32 (MFMA-bound) runs best with ~4096 chunks per launch.  16-bit inputs are bound by
+    their gathers, and run best when the launch is one wave of workgroups that all stay
+    resident: 128 x 128 channel tiles hold two workgroups per CU, and a launch just above that
+    pays a second, mostly idle wave (planes + spheres, 6.7M pairs, 128 -> 128 on MI355X: 462
+    chunks 0.58 ms, 525 chunks 0.81 ms).  Narrower tiles measured best at 3 (64-channel) and 4
+    (32-channel) workgroups per CU.  Workgroups per chunk = channel tiles of the layer."""
+    if dtype in (torch.bfloat16, torch.float16):
+        if n_cu <= 0:
+            n_cu = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count \
+                if torch.cuda.is_available() else 256
+        tm, tn = _wgrad_tile(cin), _wgrad_tile(cout)
+        tiles = -(-cin // tm) * -(-cout // tn)
+        per_cu = 1.75 if (tm, tn) == (128, 128) else (3.0 if max(tm, tn) >= 64 else 4.0)
+        target = max(1, int(per_cu * n_cu / tiles))
+    else:
+        target = 4096
     c = -(-max(n_pairs, 1) // target)
     c = -(-c // 64) * 64
-    return int(min(16384, max(256, c)))
+    return int(max(256, c))
 
 
 def check_conv_args(feats: torch.Tensor, weight: torch.Tensor, n_in: int, transposed: bool) -> None:
@@ -124,7 +143,7 @@ class SparseConvFunction(torch.autograd.Function):
         if ctx.needs_input_grad[1]:
             pairs, sizes = pack._pairs()
             gk = ext.wgrad(feats.contiguous(), grad_out, pairs, sizes, transposed,
-                           wgrad_chunk(pairs.shape[0], feats.dtype))
+                           wgrad_chunk(pairs.shape[0], feats.dtype, feats.shape[1], grad_out.shape[1]))
             gw = kio_to_weight(gk.to(kio.dtype), pack.kernel_size, transposed)
         return gin, gw, gb, None, None, None
 

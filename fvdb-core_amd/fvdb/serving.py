@@ -267,12 +267,15 @@ class UNetExecutor:
         self.fvdb = fvdb
         self.device = torch.device(device)
         self.dtype = dtype
-        self.host_results = bool(host_results)  # finish() copies each result to host memory
+        self.host_results = bool(host_results)  # results returned in pinned host memory
         self.model = model
         self.levels = len(model.channels)
         self.cin = in_channels or model.enc[0][0].in_channels
         self.ingest_stream = torch.cuda.Stream(device=self.device)
         self.compute_stream = torch.cuda.Stream(device=self.device)
+        # results go to pinned host memory on a stream of their own, so a batch's copy overlaps
+        # the next batch's compute (every stream at the default priority)
+        self.copy_stream = torch.cuda.Stream(device=self.device) if self.host_results else None
 
         def pack(conv):
             w = conv.weight.detach().to(self.device, dtype)
@@ -411,14 +414,20 @@ class UNetExecutor:
                 tp = r.state["topo"]
                 for t in [r.state["x"]] + tp.conv + tp.down + tp.up:
                     t.record_stream(self.compute_stream)
+        if self.host_results:
+            with torch.cuda.stream(self.copy_stream):
+                self.copy_stream.wait_event(done)
+                host = torch.empty(x.shape, dtype=x.dtype, pin_memory=True)
+                host.copy_(x, non_blocking=True)
+                x.record_stream(self.copy_stream)
+                done = torch.cuda.Event()
+                done.record(self.copy_stream)
+            x = host
         return {"event": done, "out": x, "offs": offs[0]}
 
     def is_done(self, handle) -> bool:
         return handle["event"].query()
 
     def finish(self, handle) -> List[torch.Tensor]:
-        o, out = handle["offs"], handle["out"]
-        if self.host_results:
-            host = out.to("cpu")
-            return [host[o[i]:o[i + 1]] for i in range(len(o) - 1)]
+        o, out = handle["offs"], handle["out"]  # out is in pinned host memory with host_results
         return [out[o[i]:o[i + 1]] for i in range(len(o) - 1)]

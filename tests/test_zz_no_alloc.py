@@ -74,7 +74,7 @@ def test_noalloc_wgrad_equals_workspace_wgrad(dtype, cin, cout):
     assert lib.fvdbk_conv_wgrad_scratch_bytes() == 64 << 20
     for n_points, seed in ((400_000, 3), (60_000, 4)):  # the second call is smaller
         pairs, sizes, x, go = _problem(fvdb, n_points, seed, cin, cout, dtype)
-        chunk = fvdb._autograd.wgrad_chunk(sum(sizes), dtype)
+        chunk = fvdb._autograd.wgrad_chunk(sum(sizes), dtype, cin, cout)
         koff, cstart, total = _tables(sizes, chunk)
         print(f"[no-alloc] {dtype} {cin}x{cout}: {x.shape[0]} voxels, {sum(sizes)} pairs, {total} chunks",
               flush=True)
