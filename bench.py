@@ -314,8 +314,9 @@ def main() -> int:
             tr = json.load(f)
         roof["traffic"] = tr["hbm_bytes_per_launch"]
         roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
-    roof["kernel"] = ("k_conv_tile_dma<bf16, LX=3, NW=8, KT=27> (conv A, 64->64, 3x3x3: 256-row workgroups, weights staged "
-                      "global->LDS, gathers 3 taps ahead, 4 waves/SIMD)")
+    roof["kernel"] = ("k_conv_tile_dma<bf16, LX=3, NW=8, KT=27, MI=4, NCC=1, WSKIP> (conv A, 64->64, 3x3x3: 256-row "
+                      "workgroups, weights staged global->LDS, gathers 3 taps ahead, 4 waves/SIMD, wavefronts skip "
+                      "the MFMAs of taps none of their rows has)")
     roof["avg_launch_ms"] = k_ms
     roof["algorithmic_flops"] = flops_a
     roof["algorithmic_bytes"] = bytes_a
@@ -476,7 +477,8 @@ def run_diff(torch, fvdb, dev, args):
         fwd = lambda: ext.implicit_gemm(x, a_f, pack._map_fwd(), N, of)  # noqa: E731
         dgr = lambda: ext.implicit_gemm(R, kio, pack._map_bwd(), N, ob)  # noqa: E731
         chunk = wgrad_chunk(P, dt, 128, 128)
-        wgr = lambda: ext.wgrad(x, R, pairs, sizes, False, chunk)  # noqa: E731
+        koff, cstart, total = pack._wgrad_tables(chunk)
+        wgr = lambda: ext.wgrad_tabled(x, R, pairs, koff, cstart, total, False, chunk)  # noqa: E731
         t = {"fwd_ms": _median_ms(torch, fwd, 10), "dgrad_ms": _median_ms(torch, dgr, 10),
              "wgrad_ms": _median_ms(torch, wgr, 10)}
         t["tflops"] = {k[:-3]: F / (v * 1e-3) / 1e12 for k, v in list(t.items())}

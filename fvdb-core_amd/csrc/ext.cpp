@@ -495,6 +495,36 @@ at::Tensor wgrad(at::Tensor in, at::Tensor gout, at::Tensor pairs, at::Tensor nb
     return gw;
 }
 
+// The same with the split-K tables already on the device (k_offsets int64 [K+1], chunk_start
+// int32 [K+1], `total` chunks): no host work or host-to-device copy per call.
+at::Tensor wgrad_tabled(at::Tensor in, at::Tensor gout, at::Tensor pairs, at::Tensor koff_t, at::Tensor cs_t,
+                        int64_t total, bool transposed, int64_t chunk) {
+    require_hip(in, "features");
+    TORCH_CHECK_TYPE(in.scalar_type() == gout.scalar_type(), "features and gradients must have the same dtype");
+    TORCH_CHECK_VALUE(in.dim() == 2 && gout.dim() == 2, "features and gradients must be 2-D");
+    TORCH_CHECK_VALUE(pairs.dim() == 2 && pairs.size(1) == 2 && pairs.scalar_type() == at::kInt,
+                      "pairs must be int32 [P, 2]");
+    TORCH_CHECK_VALUE(koff_t.is_cuda() && koff_t.scalar_type() == at::kLong && cs_t.is_cuda() &&
+                          cs_t.scalar_type() == at::kInt && koff_t.numel() == cs_t.numel() && koff_t.numel() >= 2,
+                      "k_offsets must be device int64 [K+1] and chunk_start device int32 [K+1]");
+    in = in.contiguous();
+    gout = gout.contiguous();
+    pairs = pairs.contiguous();
+    c10::hip::HIPGuard guard(in.device().index());
+    const int64_t K = koff_t.numel() - 1;
+    const fvdbk_dtype dt = to_dtype(in.scalar_type());
+    auto dev = in.options();
+    const size_t ws_bytes = fvdbk_conv_wgrad_workspace_size(dt, total, (int32_t)in.size(1), (int32_t)gout.size(1));
+    at::Tensor ws = at::empty({(int64_t)((ws_bytes + 7) / 8)}, dev.dtype(at::kLong));
+    at::Tensor gw = at::empty({K, in.size(1), gout.size(1)}, dev.dtype(dt == FVDBK_F64 ? at::kDouble : at::kFloat));
+    check_rc(fvdbk_conv_wgrad(dt, in.data_ptr(), (int32_t)in.size(1), gout.data_ptr(), (int32_t)gout.size(1),
+                              pairs.data_ptr<int32_t>(), koff_t.data_ptr<int64_t>(), cs_t.data_ptr<int32_t>(),
+                              (int32_t)K, (int32_t)chunk, total, transposed ? 1 : 0, ws.data_ptr(), ws_bytes,
+                              gw.data_ptr(), stream_of(in)),
+             "conv_wgrad");
+    return gw;
+}
+
 std::string version() { return fvdbk_version(); }
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
@@ -515,4 +545,5 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
           pybind11::arg("relu") = false);
     m.def("row_order", &row_order, py::arg("map"), py::arg("n_rows"), py::arg("window") = 0);
     m.def("wgrad", &wgrad);
+    m.def("wgrad_tabled", &wgrad_tabled);
 }

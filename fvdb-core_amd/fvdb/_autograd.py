@@ -141,9 +141,10 @@ class SparseConvFunction(torch.autograd.Function):
                 dmap, order, n_in = pack._map_fwd(), pack._order_for(True, cg, dt, co), pack.target_grid.total_voxels
             gin = ext.implicit_gemm(grad_out, kio, dmap, n_in, order)
         if ctx.needs_input_grad[1]:
-            pairs, sizes = pack._pairs()
-            gk = ext.wgrad(feats.contiguous(), grad_out, pairs, sizes, transposed,
-                           wgrad_chunk(pairs.shape[0], feats.dtype, feats.shape[1], grad_out.shape[1]))
+            pairs, _ = pack._pairs()
+            chunk = wgrad_chunk(pairs.shape[0], feats.dtype, feats.shape[1], grad_out.shape[1])
+            koff, cstart, total = pack._wgrad_tables(chunk)
+            gk = ext.wgrad_tabled(feats.contiguous(), grad_out, pairs, koff, cstart, total, transposed, chunk)
             gw = kio_to_weight(gk.to(kio.dtype), pack.kernel_size, transposed)
         return gin, gw, gb, None, None, None
 

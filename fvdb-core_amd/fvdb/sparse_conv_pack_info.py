@@ -114,6 +114,7 @@ class SparseConvPackInfo:
         self._pairs_t: Optional[torch.Tensor] = None
         self._sizes_t: Optional[torch.Tensor] = None
         self._sizes_host: Optional[torch.Tensor] = None
+        self._wg_tables = {}
         self._gs_built = False
         self._use_me = False
         self._igemm = None  # dict of build flags
@@ -186,6 +187,23 @@ class SparseConvPackInfo:
             m = self._map_fwd() if fwd else self._map_bwd()
             self._ord_auto[fwd] = ext.row_order(m, n, AUTO_ORDER_WINDOW)
         return self._ord_auto[fwd]
+
+    def _wgrad_tables(self, chunk: int):
+        """Device split-K tables of the weight gradient for `chunk` pairs per chunk: pair offsets
+        per tap (int64 [K+1]), first chunk per tap (int32 [K+1]) and the chunk count; built once
+        per chunk size from the host pair counts."""
+        if chunk not in self._wg_tables:
+            _, sizes = self._pairs()
+            s = sizes.tolist()
+            koff = [0]
+            cst = [0]
+            for n in s:
+                koff.append(koff[-1] + n)
+                cst.append(cst[-1] + -(-n // chunk))
+            dev = self._src.device
+            self._wg_tables[chunk] = (torch.tensor(koff, dtype=torch.int64).to(dev),
+                                      torch.tensor(cst, dtype=torch.int32).to(dev), cst[-1])
+        return self._wg_tables[chunk]
 
     def _pairs(self):
         if self._pairs_t is None:
