@@ -22,7 +22,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("FVDBK_ARCH", "gfx950")
 
 HIP_SOURCES = ["fvdbk_index.hip", "fvdbk_conv.hip", "fvdbk_conv_tile_bf16.hip", "fvdbk_conv_tile_f16.hip",
-               "fvdbk_conv_tile_f32.hip", "fvdbk_pool.hip"]
+               "fvdbk_conv_tile_f32.hip", "fvdbk_pool.hip", "fvdbk_norm.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",

@@ -525,6 +525,70 @@ at::Tensor wgrad_tabled(at::Tensor in, at::Tensor gout, at::Tensor pairs, at::Te
     return gw;
 }
 
+// GroupNorm over each grid's voxels: (y, mean, rstd).  offsets: device int64 [B+1]; counts: host
+// int64 [B] (the same grid sizes, known on the host); jidx: device int32 [n] or empty for B = 1.
+std::vector<at::Tensor> group_norm_forward(at::Tensor x, at::Tensor offsets, at::Tensor counts, at::Tensor jidx,
+                                           int64_t G, c10::optional<at::Tensor> gamma, c10::optional<at::Tensor> beta,
+                                           double eps, bool relu) {
+    require_hip(x, "features");
+    TORCH_CHECK_VALUE(x.dim() == 2, "features must have shape [N, C]");
+    TORCH_CHECK_TYPE(x.is_floating_point(), "features must have a floating point type");
+    x = x.contiguous();
+    counts = counts.to(at::kLong).contiguous();
+    const int64_t B = counts.numel(), C = x.size(1), n = x.size(0);
+    TORCH_CHECK_VALUE(offsets.numel() == B + 1 && offsets.is_cuda(), "offsets must be device int64 [B+1]");
+    c10::hip::HIPGuard guard(x.device().index());
+    const fvdbk_dtype dt = to_dtype(x.scalar_type());
+    auto acc = x.options().dtype(dt == FVDBK_F64 ? at::kDouble : at::kFloat);
+    at::Tensor g, b;
+    if (gamma.has_value() && gamma->defined()) g = gamma->to(x.scalar_type()).contiguous();
+    if (beta.has_value() && beta->defined()) b = beta->to(x.scalar_type()).contiguous();
+    const int64_t nb = fvdbk_group_norm_blocks(counts.data_ptr<int64_t>(), (int32_t)B);
+    const size_t wsb = fvdbk_group_norm_workspace_size(dt, nb, (int32_t)B, (int32_t)C, (int32_t)G);
+    at::Tensor ws = at::empty({(int64_t)((wsb + 7) / 8)}, x.options().dtype(at::kLong));
+    at::Tensor y = at::empty_like(x), mean = at::empty({B, G}, acc), rstd = at::empty({B, G}, acc);
+    at::Tensor jx = jidx.numel() ? jidx.to(at::kInt).contiguous() : jidx;
+    at::Tensor off = offsets.to(at::kLong).contiguous();
+    check_rc(fvdbk_group_norm_forward(dt, x.data_ptr(), n, (int32_t)C, (int32_t)G, off.data_ptr<int64_t>(),
+                                      jx.numel() ? jx.data_ptr<int32_t>() : nullptr, (int32_t)B, nb,
+                                      g.defined() ? g.data_ptr() : nullptr, b.defined() ? b.data_ptr() : nullptr, eps,
+                                      relu ? FVDBK_CONV_RELU : 0u, y.data_ptr(), mean.data_ptr(), rstd.data_ptr(),
+                                      ws.data_ptr(), wsb, stream_of(x)),
+             "group_norm_forward");
+    return {y, mean, rstd};
+}
+
+// (dx, dgamma, dbeta); y is the forward output when the forward applied a ReLU (else empty).
+std::vector<at::Tensor> group_norm_backward(at::Tensor x, at::Tensor dy, at::Tensor y, at::Tensor offsets,
+                                            at::Tensor counts, at::Tensor jidx, int64_t G, at::Tensor mean,
+                                            at::Tensor rstd, c10::optional<at::Tensor> gamma) {
+    require_hip(x, "features");
+    x = x.contiguous();
+    dy = dy.to(x.scalar_type()).contiguous();
+    counts = counts.to(at::kLong).contiguous();
+    const int64_t B = counts.numel(), C = x.size(1), n = x.size(0);
+    c10::hip::HIPGuard guard(x.device().index());
+    const fvdbk_dtype dt = to_dtype(x.scalar_type());
+    auto acc = x.options().dtype(dt == FVDBK_F64 ? at::kDouble : at::kFloat);
+    at::Tensor g;
+    if (gamma.has_value() && gamma->defined()) g = gamma->to(x.scalar_type()).contiguous();
+    const int64_t nb = fvdbk_group_norm_blocks(counts.data_ptr<int64_t>(), (int32_t)B);
+    const size_t wsb = fvdbk_group_norm_workspace_size(dt, nb, (int32_t)B, (int32_t)C, (int32_t)G);
+    at::Tensor ws = at::empty({(int64_t)((wsb + 7) / 8)}, x.options().dtype(at::kLong));
+    at::Tensor dx = at::empty_like(x), dg = at::empty({C}, acc), db = at::empty({C}, acc);
+    at::Tensor jx = jidx.numel() ? jidx.to(at::kInt).contiguous() : jidx;
+    at::Tensor off = offsets.to(at::kLong).contiguous();
+    at::Tensor yy = y.numel() ? y.contiguous() : y;
+    check_rc(fvdbk_group_norm_backward(dt, x.data_ptr(), dy.data_ptr(), yy.numel() ? yy.data_ptr() : nullptr, n,
+                                       (int32_t)C, (int32_t)G, off.data_ptr<int64_t>(),
+                                       jx.numel() ? jx.data_ptr<int32_t>() : nullptr, (int32_t)B, nb,
+                                       mean.contiguous().data_ptr(), rstd.contiguous().data_ptr(),
+                                       g.defined() ? g.data_ptr() : nullptr, dx.data_ptr(), dg.data_ptr(),
+                                       db.data_ptr(), ws.data_ptr(), wsb, stream_of(x)),
+             "group_norm_backward");
+    return {dx, dg, db};
+}
+
 std::string version() { return fvdbk_version(); }
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
@@ -546,4 +610,6 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("row_order", &row_order, py::arg("map"), py::arg("n_rows"), py::arg("window") = 0);
     m.def("wgrad", &wgrad);
     m.def("wgrad_tabled", &wgrad_tabled);
+    m.def("group_norm_forward", &group_norm_forward);
+    m.def("group_norm_backward", &group_norm_backward);
 }

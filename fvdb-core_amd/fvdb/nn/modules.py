@@ -312,14 +312,43 @@ class BatchNorm(nn.BatchNorm1d):
 
 @fvnn_module
 class GroupNorm(nn.GroupNorm):
-    def forward(self, input: VDBTensor) -> VDBTensor:
+    """``torch.nn.GroupNorm`` per grid of a VDBTensor (``fvdb/nn/modules.py:407-432``): statistics
+    per (grid, group) over the grid's voxels.  Runs on the native group-norm kernels (two
+    passes over the batch, deterministic reductions); ``fuse_relu=True`` applies a ReLU in the
+    normalize pass."""
+
+    def forward(self, input: VDBTensor, *, fuse_relu: bool = False) -> VDBTensor:
+        from .._norm import group_norm
+
         C = input.data.jdata.size(1)
-        assert C == self.num_channels
-        flat, offs = input.data.jdata, input.data.joffsets.tolist()
-        out = torch.empty_like(flat)
-        for b in range(input.grid.grid_count):
-            feat = flat[offs[b]:offs[b + 1]]
-            if feat.size(0) != 0:
-                f = super().forward(feat.transpose(0, 1).contiguous().reshape(1, C, -1))
-                out[offs[b]:offs[b + 1]] = f.reshape(C, -1).transpose(0, 1)
+        assert C == self.num_channels, "Input feature should have the same number of channels as GroupNorm"
+        out = group_norm(input.data.jdata, input.grid, self.num_groups, self.weight, self.bias, self.eps,
+                         relu=fuse_relu)
         return VDBTensor(input.grid, input.grid.jagged_like(out), input.kmap)
+
+
+def fuse_conv_bn(conv: "SparseConv3d", bn: "BatchNorm") -> "SparseConv3d":
+    """A SparseConv3d equal to ``bn(conv(x))`` with the batch norm in inference mode folded into
+    the convolution: W' = W * g / sqrt(var + eps) per output channel, b' = (b - mean) * g /
+    sqrt(var + eps) + beta.  The folded bias runs in the convolution's epilogue (with a fused
+    ReLU when the caller passes ``fuse_relu=True``), so the normalization costs no pass of its
+    own."""
+    if bn.training or bn.running_mean is None:
+        raise ValueError("fuse_conv_bn folds a batch norm in inference mode (bn.eval(), running statistics)")
+    if bn.num_features != conv.out_channels:
+        raise ValueError("the batch norm's features must be the convolution's output channels")
+    fused = SparseConv3d(conv.in_channels, conv.out_channels, conv.kernel_size, conv.stride, bias=True,
+                         transposed=conv.transposed).to(device=conv.weight.device, dtype=conv.weight.dtype)
+    fused.backend = conv.backend
+    with torch.no_grad():
+        scale = bn.weight.double() / torch.sqrt(bn.running_var.double() + bn.eps) if bn.affine else \
+            1.0 / torch.sqrt(bn.running_var.double() + bn.eps)
+        shift = (bn.bias.double() if bn.affine else 0.0) - bn.running_mean.double() * scale
+        w = conv.weight.double()
+        # output channels are dim 0, or dim 1 for a transposed conv's [I, O, k...] weight
+        shape = [1] * w.dim()
+        shape[1 if conv.transposed else 0] = -1
+        fused.weight.copy_((w * scale.reshape(shape)).to(conv.weight.dtype))
+        b = conv.bias.double() if conv.bias is not None else torch.zeros_like(scale)
+        fused.bias.copy_((b * scale + shift).to(conv.weight.dtype))
+    return fused

@@ -315,6 +315,35 @@ int fvdbk_conv_wgrad(fvdbk_dtype dtype, const void* in, int32_t c_in, const void
                      size_t workspace_bytes, void* gw, hipStream_t stream);
 
 /* ---------------------------------------------------------------------------
+ * Group normalization over the voxels of each grid (replaces the per-grid loop of
+ * fvdb/nn/modules.py:407-432, which runs torch.nn.GroupNorm on each grid's [1, C, n] view).
+ *
+ * x, y [n][C] in `dtype`, grids contiguous: rows offsets[b] .. offsets[b+1] (device int64
+ * [n_grids+1]); jidx (device int32 [n], the grid of each row) may be NULL for one grid.
+ * Channels form G groups of C/G.  mean / rstd are [n_grids][G] in fp32 (fp64 for FVDBK_F64):
+ * the mean and 1/sqrt(biased variance + eps) over each (grid, group)'s rows x channels.
+ *   y = (x - mean) * rstd * gamma[c] + beta[c]   (gamma / beta [C] in dtype, or NULL),
+ * then max(., 0) with FVDBK_CONV_RELU.  n_blocks = fvdbk_group_norm_blocks(host counts):
+ * the statistics are reduced over blocks of FVDBK_GN_ROWS rows of one grid, each merged in
+ * a fixed order, so results are deterministic and a grid's rows do not depend on the other
+ * grids of the batch.  The backward pass takes dy (and y when the forward applied the
+ * ReLU, else NULL) and writes dx [n][C] and, when not NULL, dgamma / dbeta [C] in fp32
+ * (fp64 for FVDBK_F64).  The workspace (fvdbk_group_norm_workspace_size) is device scratch.
+ * ------------------------------------------------------------------------- */
+#define FVDBK_GN_ROWS 1024
+int64_t fvdbk_group_norm_blocks(const int64_t* counts, int32_t n_grids);
+size_t fvdbk_group_norm_workspace_size(fvdbk_dtype dtype, int64_t n_blocks, int32_t n_grids, int32_t channels,
+                                       int32_t groups);
+int fvdbk_group_norm_forward(fvdbk_dtype dtype, const void* x, int64_t n, int32_t C, int32_t G, const int64_t* offsets,
+                             const int32_t* jidx, int32_t n_grids, int64_t n_blocks, const void* gamma,
+                             const void* beta, double eps, uint32_t flags, void* y, void* mean, void* rstd, void* ws,
+                             size_t ws_bytes, hipStream_t stream);
+int fvdbk_group_norm_backward(fvdbk_dtype dtype, const void* x, const void* dy, const void* y, int64_t n, int32_t C,
+                              int32_t G, const int64_t* offsets, const int32_t* jidx, int32_t n_grids,
+                              int64_t n_blocks, const void* mean, const void* rstd, const void* gamma, void* dx,
+                              void* dgamma, void* dbeta, void* ws, size_t ws_bytes, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
  * Weight gradient that allocates nothing.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  Same arguments and result as fvdbk_conv_wgrad
