@@ -12,6 +12,7 @@ from .jagged_tensor import JaggedTensor, jcat_tensors
 from .grid_batch import GridBatch
 from .sparse_conv_pack_info import CUTLASS, GATHER_SCATTER, IGEMM, LGGS, ConvPackBackend, SparseConvPackInfo
 from . import nn
+from .io import load_gridbatch, save_gridbatch
 
 
 class _Config:
@@ -50,4 +51,4 @@ def jcat(things_to_cat, dim=None):
 __version__ = "0.2.1+mi355x"
 
 __all__ = ["GridBatch", "JaggedTensor", "SparseConvPackInfo", "ConvPackBackend", "config", "jcat", "nn",
-           "GATHER_SCATTER", "IGEMM", "CUTLASS", "LGGS"]
+           "GATHER_SCATTER", "IGEMM", "CUTLASS", "LGGS", "load_gridbatch", "save_gridbatch"]

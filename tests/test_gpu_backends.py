@@ -293,3 +293,41 @@ def test_c_abi_kernel_map_and_contraction():
         e = rel_err(out, oracle.sparse_conv(xd.double().cpu(), w.to(dt).double(), km, N, 1))
         print(f"[abi] {dt}: rel err {e:.2e}", flush=True)
         assert e <= tol
+
+
+def test_save_load_gridbatch_round_trip(tmp_path):
+    """``save_gridbatch`` / ``load_gridbatch`` (``fvdb/grid_batch.py:2056-2200``): the grids come back in
+    the same voxel order with their voxel sizes, origins, names and data (bf16 bits and fp32), all
+    or by index / indices / name / names; a NanoVDB file is rejected."""
+    fvdb = _fvdb()
+    grid = point_grid(fvdb, n_grids=3, n=2000, seed=12)
+    grid = fvdb.GridBatch.from_ijk(grid.ijk, voxel_sizes=[[0.1, 0.1, 0.1], [0.2, 0.2, 0.3], [0.05, 0.05, 0.05]],
+                                   origins=[[0.0, 1.0, 2.0], [3.0, 4.0, 5.0], [-1.0, 0.0, 1.0]])
+    g = torch.Generator().manual_seed(2)
+    for dtype, compressed in ((torch.bfloat16, False), (torch.float32, True)):
+        data = torch.randn(grid.total_voxels, 5, generator=g).to("cuda", dtype)
+        path = str(tmp_path / f"grids_{dtype}.nvdb")
+        fvdb.save_gridbatch(path, grid, grid.jagged_like(data), names=["a", "b", "c"], compressed=compressed)
+        g2, d2, names = fvdb.load_gridbatch(path)
+        assert names == ["a", "b", "c"]
+        assert torch.equal(g2.ijk.jdata, grid.ijk.jdata) and torch.equal(g2.joffsets, grid.joffsets)
+        assert torch.equal(d2.jdata, data)
+        assert torch.allclose(g2.voxel_sizes, grid.voxel_sizes) and torch.allclose(g2.origins, grid.origins)
+        jo = grid.joffsets.cpu().tolist()
+        for kw, sel in (({"index": 1}, [1]), ({"indices": [2, 0]}, [2, 0]), ({"name": "c"}, [2]),
+                        ({"names": ["b", "a"]}, [1, 0])):
+            gs, ds, ns = fvdb.load_gridbatch(path, **kw)
+            assert ns == [["a", "b", "c"][i] for i in sel]
+            assert torch.equal(gs.ijk.jdata, torch.cat([grid.ijk.jdata[jo[i]:jo[i + 1]] for i in sel]))
+            assert torch.equal(ds.jdata, torch.cat([data[jo[i]:jo[i + 1]] for i in sel]))
+    # grids only: zero data channels
+    path = str(tmp_path / "plain.nvdb")
+    fvdb.save_gridbatch(path, grid, name="room")
+    g3, d3, n3 = fvdb.load_gridbatch(path)
+    assert n3 == ["room"] * 3 and tuple(d3.jdata.shape) == (grid.total_voxels, 0)
+    with pytest.raises(ValueError):
+        fvdb.load_gridbatch(path, index=0, name="room")
+    nano = tmp_path / "real.nvdb"
+    nano.write_bytes(b"NanoVDB0" + bytes(64))
+    with pytest.raises(NotImplementedError):
+        fvdb.load_gridbatch(str(nano))
