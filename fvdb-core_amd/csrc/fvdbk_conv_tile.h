@@ -25,6 +25,15 @@ namespace fvdbk {
 
 constexpr int TILE_MAX_K = 32;  // taps whose map tile is staged in LDS (3^3, 2^3, ...)
 
+// f(integral_constant<int, I>) for I = B .. E-1, in order (compile-time loop indices)
+template <int B, int E, typename Fn>
+__device__ inline void static_for(Fn&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 template <typename T, int E>
 struct StoreVec {
     typedef T type __attribute__((ext_vector_type(E)));
@@ -570,12 +579,21 @@ void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict_
                 bb[j][ch] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(xr, off + ch * 64, 0, 0));
         }
     };
-    // Loads issued after the staging of tap k+1, up to the end of tap k: issue order is
-    // stage 0, stage 1, gathers 0..LX-1, then per tap t: stage t+2, gathers t+LX.
-    auto after_stage = [&](int v) -> int {
+    // The wait at the end of step v: issue order is stage 0, stage 1, gathers 0..LX-1 (all
+    // complete before the loop), then per step t: stage t+2, gathers t+LX.  It lets through
+    // only the loads issued after both the staging of step v+1 (its weights must have landed
+    // before the barrier) and the gathers of step v+1 (issued in step v+1-LX), so that step
+    // v+1's MFMAs find their operands complete: with direct-to-LDS copies in flight the
+    // compiler's own wait for gathered registers would be vmcnt(0), which would also wait for
+    // the loads issued for later steps.
+    auto step_wait = [&](int v) -> int {
         auto G = [&](int tt) { return tt + LX < KV ? GPT : 0; };
         auto D = [&](int tt) { return tt + 2 < KV ? SQ : 0; };
-        return v == 0 ? GPT * LX + D(0) + G(0) : G(v - 1) + D(v) + G(v);
+        const int st = v == 0 ? D(0) + G(0) : G(v - 1) + D(v) + G(v);
+        if (v + 1 < LX) return st;
+        int ga = 0;
+        for (int tt = v + 2 - LX; tt <= v; ++tt) ga += D(tt) + G(tt);
+        return ga < st ? ga : st;
     };
 
     __syncthreads();  // map tile visible
@@ -613,7 +631,7 @@ void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict_
             // step v+1's weights have landed in every wavefront; nobody reads step v's buffer
             // again before stage(v+3) overwrites it
             __builtin_amdgcn_sched_barrier(0);
-            wait_vm_lds(after_stage(v));
+            wait_vm_lds(step_wait(v));
             wg_barrier();
         }
     }
@@ -633,6 +651,215 @@ void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict_
                                    nullptr, n_rows, M, m0, bias, relu);
         store_tile_slice<T, 4, NI>(Y, *reinterpret_cast<const Half*>(&acc[4][0]), slice, lane, row0 + wave * NI * 16,
                                    nullptr, n_rows, M, m0 + 64, bias, relu);
+    }
+}
+
+// k_conv_tile_dma over rows taken through a tap-mask ordering `perm` (row i of the tile is
+// output row perm[row0 + i]), with compacted steps: the prologue ORs the tile's map
+// entries into a tap mask, and the pipeline runs only the (tap, chunk) steps of the taps
+// that at least one of the tile's rows uses, so a skipped tap costs neither its weight
+// staging, its gathers, its barrier nor its MFMAs; inside a step a wavefront none of whose
+// 32 rows has the tap skips its MFMAs as well.  Same wavefront shape, LDS layout and
+// accumulation order (taps ascending, chunks ascending) as k_conv_tile_dma, and a skipped
+// step would have added exact zeros, so the output is bit-identical to the tree-order
+// kernel for finite weights, whatever the order.
+// The step count is known only at run time: the loop body is U = lcm(3, LX+1) steps (a
+// multiple of the 3-entry weight ring and the LX+1-entry gather ring, so ring slots stay
+// compile-time), unguarded while the group and the loads it issues lie inside the step
+// sequence, and guarded in the last groups, where the per-step load count, and with it the
+// wait before each barrier, depends on how many steps remain.  WPE waves per SIMD: 4
+// (128 registers per lane) or 2 (256, room for deeper gather lookahead).
+template <typename T, int LX, int NW, int KT, int MI, int NCC, int WPE = 4>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+void k_conv_tile_dma_perm(const T* __restrict__ X, int64_t n_in, const T* __restrict__ A,
+                          const int32_t* __restrict__ map, const int32_t* __restrict__ perm, int64_t n_rows, int M,
+                          int n_mt, int64_t n_tiles, T* __restrict__ Y, const T* __restrict__ bias, int relu) {
+    static_assert(sizeof(T) == 2, "16-bit features");
+    static_assert(KT <= 32, "tap masks are 32 bits");
+    using F = Frag<T>;
+    using V = typename F::vec;
+    constexpr int NI = 2, CC = 64, CIN = CC * NCC, U = LX == 3 ? 12 : 6;
+    constexpr int RT = NW * NI * 16;
+    constexpr int WT = MI * 16 * CC;
+    constexpr int NCH = CC / F::CH;
+    constexpr int GPT = NI * NCH;
+    constexpr int SQ = 2 * MI / NW;
+    constexpr int OUT_SLICE = NI * 16 * (4 * 16 + 8);
+    static_assert(NW == 4 || NW == 8, "4 or 8 wavefronts");
+    static_assert(MI == 4 || MI == 8, "64 or 128 output channels");
+    static_assert(NCC == 1 || NCC == 2, "64 or 128 input channels");
+    static_assert(LX >= 1 && LX <= 3, "gathers one to three steps ahead");
+    static_assert(U % (LX + 1) == 0 && U % 3 == 0, "rings must divide the unrolled step count");
+    static_assert(OUT_SLICE <= WT && (MI == 4 || 3 * OUT_SLICE <= WT), "output slices fit the weight buffers");
+    static_assert(WT * sizeof(T) == (MI == 4 ? sizeof(dma_ring0) : sizeof(dma8_ring0)), "ring buffer size");
+    extern __shared__ __attribute__((aligned(16))) unsigned char tile_smem[];
+    __shared__ unsigned int s_tmask;
+    T* const sW0 = reinterpret_cast<T*>(MI == 4 ? dma_ring0 : dma8_ring0);
+    T* const sW1 = reinterpret_cast<T*>(MI == 4 ? dma_ring1 : dma8_ring1);
+    T* const sW2 = reinterpret_cast<T*>(MI == 4 ? dma_ring2 : dma8_ring2);
+    int32_t* sMap = reinterpret_cast<int32_t*>(tile_smem);
+    auto wring = [&](auto rc) -> T* {
+        constexpr int r = decltype(rc)::value % 3;
+        return r == 0 ? sW0 : (r == 1 ? sW1 : sW2);
+    };
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int64_t per = (n_tiles + 7) / 8;
+    const int64_t t = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (t >= n_tiles) return;
+    const int64_t row0 = (t / n_mt) * RT;
+    const int m0 = (int)(t % n_mt) * (MI * 16);
+    if (tid == 0) s_tmask = 0u;
+    __syncthreads();
+    {
+        unsigned int tm = 0u;
+        for (int e = tid; e < RT * KT; e += 64 * NW) {
+            const int i = e / KT, k = e - i * KT;
+            const int64_t r = row0 + i;
+            const int32_t v = r < n_rows ? map[(int64_t)perm[r] * KT + k] : -1;
+            sMap[e] = v;
+            if (v >= 0) tm |= 1u << k;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) tm |= __shfl_xor(tm, off);
+        if (lane == 0 && tm) atomicOr(&s_tmask, tm);
+    }
+    __syncthreads();  // map tile and tap mask visible
+    // the weight and gather streams walk the tile's taps with uniform bit iterators: the
+    // lowest set bit is the tap of the stream's next step, cleared after its last chunk
+    unsigned int fit = (unsigned int)__builtin_amdgcn_readfirstlane((int)s_tmask), git = fit;
+    const int kv = __popc(fit) * NCC;
+    auto next_tap = [](unsigned int& it, int v) -> int {
+        const int tap = __builtin_ctz(it);
+        if ((v & (NCC - 1)) == NCC - 1) it &= it - 1u;
+        return tap;
+    };
+
+    floatx4 acc[MI][NI];
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NI; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    // staging: the MI*16 weight rows are 2*MI blocks of 8; wavefront w copies blocks
+    // w*SQ .. +SQ-1, lane -> (row, swizzled 16-byte position); 32-bit element offsets
+    // over a wavefront-uniform base
+    int woff[SQ];
+#pragma unroll
+    for (int q = 0; q < SQ; ++q) {
+        const int row = (wave * SQ + q) * 8 + (lane >> 3);
+        const int seg = (lane & 7) ^ (lane >> 3);
+        woff[q] = (m0 + row) * CIN + seg * 8;
+    }
+    // weights of step v into ring buffer `dst` (called once per step, in step order)
+    auto stage = [&](int v, T* dst) {
+        const int k = next_tap(fit, v), c = v & (NCC - 1);
+        const T* src = A + (int64_t)k * M * CIN + c * CC;
+        dst += wave * SQ * 512;
+#pragma unroll
+        for (int q = 0; q < SQ; ++q) lds_dma16(src + woff[q], dst + q * 512);
+    };
+    const int bytes = (int)(n_in * CIN * (int64_t)sizeof(T));
+    const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, bytes, 0x00020000);
+    constexpr int MISSING = 0x7FFFFF00;  // beyond the features (checked by the launcher): loads zeros
+    V b[LX + 1][NI][NCH];
+    bool has[LX + 1][NI];
+    // input rows of step v (called once per step, in step order)
+    auto gather = [&](int v, V (&bb)[NI][NCH], bool (&hh)[NI]) {
+        const int k = next_tap(git, v), c = v & (NCC - 1);
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+            const int32_t src = sMap[(wave * NI * 16 + j * 16 + r16) * KT + k];
+            const int off = src >= 0 ? src * (CIN * (int)sizeof(T)) + c * (CC * (int)sizeof(T)) + g * 16 : MISSING;
+            hh[j] = __any(src >= 0);
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch)
+                bb[j][ch] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(xr, off + ch * 64, 0, 0));
+        }
+    };
+    // one step; u = v mod U fixes the ring slots.  GUARD: the step may be among the last
+    // three, so the loads it issues and the wait before its barrier depend on kv.
+    auto step = [&](int v, auto uc, auto gc) {
+        constexpr int u = decltype(uc)::value;
+        constexpr bool GUARD = decltype(gc)::value;
+        if (!GUARD || v + 2 < kv) stage(v + 2, wring(std::integral_constant<int, u + 2>{}));
+        if (!GUARD || v + LX < kv) gather(v + LX, b[(u + LX) % (LX + 1)], has[(u + LX) % (LX + 1)]);
+        const T* wb = wring(uc);
+        constexpr int sl = u % (LX + 1);
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < NI; ++j) any |= has[sl][j];
+        if (any) {
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) {
+#pragma unroll
+                for (int i = 0; i < MI; ++i) {
+                    const int pos = (ch * 4 + g) ^ (r16 & 7);
+                    const V a = *reinterpret_cast<const V*>(wb + (i * 16 + r16) * CC + pos * 8);
+#pragma unroll
+                    for (int j = 0; j < NI; ++j) mfma(a, b[sl][j][ch], acc[i][j]);
+                }
+            }
+        }
+        if (!GUARD || v + 1 < kv) {
+            // step v+1's weights have landed in every wavefront and its gathered rows are
+            // complete (the same wait as k_conv_tile_dma's: only loads issued after both)
+            __builtin_amdgcn_sched_barrier(0);
+            if constexpr (!GUARD) {
+                constexpr int GA = (LX - 1) * (GPT + SQ), ST = 2 * GPT + SQ;
+                wait_vm_lds<GA < ST ? GA : ST>();
+            } else {
+                int n = (v - 1 + LX < kv ? GPT : 0) + (v + 2 < kv ? SQ : 0) + (v + LX < kv ? GPT : 0);
+                if (v + 1 >= LX) {
+                    int ga = 0;
+#pragma unroll
+                    for (int d = 2 - LX; d <= 0; ++d) ga += (v + d + 2 < kv ? SQ : 0) + (v + d + LX < kv ? GPT : 0);
+                    n = ga < n ? ga : n;
+                }
+                wait_vm_lds(n);
+            }
+            wg_barrier();
+        }
+    };
+    // U steps from v0 (a multiple of U); guarded groups stop at the last step
+    auto group = [&](int v0, auto gc) {
+        constexpr bool GUARD = decltype(gc)::value;
+        static_for<0, U>([&](auto uc) {
+            if (!GUARD || v0 + decltype(uc)::value < kv) step(v0 + decltype(uc)::value, uc, gc);
+        });
+    };
+
+    if (kv > 0) stage(0, sW0);
+    if (kv > 1) stage(1, sW1);
+#pragma unroll
+    for (int v = 0; v < LX; ++v)
+        if (v < kv) gather(v, b[v], has[v]);
+    wait_vm_lds<0>();
+    wg_barrier();
+    int v0 = 0;
+    // unguarded groups: every step v0..v0+U-1 issues its staging (v + 2 < kv) and gathers
+    // (v + LX < kv) and ends with a barrier
+    constexpr int AHEAD = LX > 2 ? LX : 2;
+#pragma unroll 1
+    for (; v0 + U - 1 + AHEAD < kv; v0 += U) group(v0, std::false_type{});
+#pragma unroll 1
+    for (; v0 < kv; v0 += U) group(v0, std::true_type{});
+    __syncthreads();  // every wavefront is done with the weight buffers and the map tile
+    if constexpr (MI == 4) {
+        T* slice = wave == 0 ? sW0
+                   : wave == 1 ? sW1
+                   : wave == 2 ? sW2
+                               : reinterpret_cast<T*>(tile_smem) + (wave - 3) * OUT_SLICE;
+        store_tile_slice<T, MI, NI>(Y, acc, slice, lane, row0 + wave * NI * 16, perm, n_rows, M, m0, bias, relu);
+    } else {
+        T* const ring = wave < 3 ? sW0 : (wave < 6 ? sW1 : sW2);
+        T* slice = ring + (wave % 3) * OUT_SLICE;
+        using Half = floatx4[4][NI];
+        store_tile_slice<T, 4, NI>(Y, *reinterpret_cast<const Half*>(&acc[0][0]), slice, lane, row0 + wave * NI * 16,
+                                   perm, n_rows, M, m0, bias, relu);
+        store_tile_slice<T, 4, NI>(Y, *reinterpret_cast<const Half*>(&acc[4][0]), slice, lane, row0 + wave * NI * 16,
+                                   perm, n_rows, M, m0 + 64, bias, relu);
     }
 }
 
@@ -718,6 +945,45 @@ static bool launch_tile_dma_wide(const T* X, int64_t n_in, const T* A, const int
     hipLaunchKernelGGL((k_conv_tile_dma<T, 3, NW, KT, 4, 2, true>), grid, dim3(64 * NW),
                        map_bytes > out_bytes ? map_bytes : out_bytes, s, X, n_in, A, map, n_rows, M, n_mt, n_tiles, Y,
                        bias, relu);
+    return true;
+}
+
+// Direct-to-LDS contraction of 16-bit 3x3x3 layers over rows in a tap-mask order
+// (k_conv_tile_dma_perm, workgroups of 8 wavefronts = 256 rows): 64 or 128 input
+// channels, outputs in multiples of 64 channels (128-channel workgroups one step ahead
+// where the output is a multiple of 128, else 64-channel workgroups two steps ahead).
+// False if the layer does not fit it.
+template <typename T>
+static bool launch_tile_dma_perm(const T* X, int64_t n_in, int cin, const T* A, const int32_t* map,
+                                 const int32_t* perm, int64_t n_rows, int K, int M, T* Y, const T* bias, int relu,
+                                 hipStream_t s) {
+    if (K != 27 || (cin != 64 && cin != 128) || M % 64 != 0 || n_in * cin * (int64_t)sizeof(T) > 0x7FFFFF00 ||
+        ((uintptr_t)Y % 16) != 0)
+        return false;
+    constexpr int NW = 8, KT = 27;
+    const size_t map_bytes = (size_t)32 * NW * KT * 4;
+    auto go = [&](auto li, auto mi, auto ncc, auto wpe) {
+        constexpr int LX = decltype(li)::value, MI = decltype(mi)::value, NCC = decltype(ncc)::value;
+        constexpr int WPE = decltype(wpe)::value;
+        const int n_mt = M / (MI * 16);
+        const int64_t n_tiles = (n_rows + 32 * NW - 1) / (32 * NW) * n_mt;
+        const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
+        const size_t out_bytes = MI == 4 ? (size_t)(NW - 3) * 2 * 16 * (64 + 8) * sizeof(T) : 0;
+        hipLaunchKernelGGL((k_conv_tile_dma_perm<T, LX, NW, KT, MI, NCC, WPE>), grid, dim3(64 * NW),
+                           map_bytes > out_bytes ? map_bytes : out_bytes, s, X, n_in, A, map, perm, n_rows, M, n_mt,
+                           n_tiles, Y, bias, relu);
+    };
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I4 = std::integral_constant<int, 4>;
+    using I8 = std::integral_constant<int, 8>;
+    if (cin == 128) {
+        if (M % 128 == 0) go(I1{}, I8{}, I2{}, I4{});
+        else go(I2{}, I4{}, I2{}, I4{});
+    } else {
+        if (M % 128 == 0) go(I1{}, I8{}, I1{}, I4{});
+        else go(I2{}, I4{}, I1{}, I4{});
+    }
     return true;
 }
 
@@ -1081,6 +1347,11 @@ inline bool conv_tile_dispatch_t(const T* X, int64_t n_in, int cin, const T* A, 
                                  const int32_t* perm, int64_t n_rows, int K, int M, T* Y, const T* bias, int relu,
                                  hipStream_t s) {
     if (K > TILE_MAX_K || ((uintptr_t)X % 16) != 0 || ((uintptr_t)A % 16) != 0) return false;
+    if constexpr (sizeof(T) == 2) {
+        // rows through a tap-mask ordering: direct-to-LDS kernel with compacted steps
+        if (perm != nullptr && launch_tile_dma_perm<T>(X, n_in, cin, A, map, perm, n_rows, K, M, Y, bias, relu, s))
+            return true;
+    }
     switch (cin) {
     case 32: return try_tile<T, 32>(X, n_in, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     case 64: return try_tile<T, 64>(X, n_in, A, map, perm, n_rows, K, M, Y, bias, relu, s);

@@ -56,23 +56,24 @@ AUTO_ORDER_WINDOW = 8192
 
 def wants_row_order(kernel_volume: int, cin: int, dtype: torch.dtype, cout: int = 0, rows: int = 0) -> bool:
     """Whether a contraction runs faster with its rows ordered by tap mask per window
-    (measured on MI355X): fp32 (MFMA-bound, so each skipped tap is saved time) and
-    16-bit inputs of 128+ channels.  Narrower 16-bit layers stay in tree order, whose
-    gathers hit cache; that includes maps with one tap per row (a 2x2x2 stride-2
-    kernel's fine-to-coarse map), where tree order computes 8 taps per row.  Large 16-bit
-    3x3x3 layers of 128 input and 64k output channels (``cout``; at least 1024 workgroups
-    of 256 ``rows`` and 128 -- or, for odd multiples of 64, 64 -- output channels) also stay
-    in tree order: the direct-to-LDS kernel runs them faster there (planes + spheres:
-    128 -> 128 0.71 vs 1.04 ms, 128 -> 64 0.51 vs 0.67 ms).  Only the kernel volumes
-    with tap-skipping kernels (27, 8) qualify; fp64 runs the scalar kernel."""
+    (measured on MI355X): fp32 (MFMA-bound, so each skipped tap is saved time), 16-bit
+    inputs of 128+ channels, and 16-bit 3x3x3 layers of 64 inputs and outputs in
+    multiples of 128 channels.  16-bit 3x3x3 layers of 64 or 128 inputs then run the
+    direct-to-LDS kernel with compacted steps (planes + spheres, 1.1M rows: 128 -> 128
+    0.73 ms windowed vs 0.76 ms in tree order, 128 -> 64 0.37 vs 0.40 ms, 64 -> 128 0.47 vs
+    1.02 ms).  Other narrow 16-bit layers stay in tree order, whose gathers hit cache
+    (3x3x3 64 -> 64 on the sphere batch: 0.73 ms tree vs 0.81 ms windowed); that includes
+    maps with one tap per row (a 2x2x2 stride-2 kernel's fine-to-coarse map), where tree
+    order computes 8 taps per row.  Only the kernel volumes with tap-skipping kernels (27,
+    8) qualify; fp64 runs the scalar kernel.  ``rows`` is accepted for callers that pass
+    it and does not change the choice."""
+    del rows
     if kernel_volume not in (8, 27) or dtype == torch.float64:
         return False
     if dtype == torch.float32:
         return True
-    if kernel_volume == 27 and cin == 128 and cout > 0 and cout % 64 == 0:
-        tiles = (rows + 255) // 256 * (cout // 128 if cout % 128 == 0 else cout // 64)
-        if tiles >= 1024:
-            return False
+    if kernel_volume == 27 and cin == 64 and cout > 0 and cout % 128 == 0:
+        return True
     return cin >= 128
 
 

@@ -480,6 +480,47 @@ def test_direct_lds_tile_bit_exact(dtype, cout, large, ks, st):
 
 
 @pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
+@pytest.mark.parametrize("cin,cout", [(64, 64), (64, 128), (128, 128), (128, 64), (128, 192), (64, 320)])
+def test_direct_lds_ordered_rows_bit_exact(dtype, cin, cout):
+    """The direct-to-LDS contraction over rows in a given order with compacted steps (16-bit
+    3x3x3, 64 or 128 input channels, outputs in multiples of 64 channels) equals the tree-order
+    result bit for bit -- same accumulation order, and a skipped step adds exact zeros --
+    through the global and windowed tap-mask orders, a random permutation and the identity,
+    with and without the fused bias + ReLU, on a ragged last tile and with a whole 256-row
+    tile of rows without neighbours (a tile with no steps), for the forward and the
+    transposed map; and the fp64 oracle within the 16-bit bound."""
+    fvdb = _fvdb()
+    from fvdb._lib import ext
+    from fvdb.utils.synthetic import sphere_points, voxelize
+
+    grid = voxelize(sphere_points(seed=6, n_points=300_000, device="cuda"), 1 / 200)
+    pack, _ = grid.sparse_conv_kernel_map(3, 1)
+    pack.build_implicit_gemm(training=True)
+    n = grid.total_voxels
+    assert n % 256 != 0 and n > 1024, n
+    print(f"[direct-lds-ordered] {dtype} {cin}->{cout} rows={n} (last tile {n % 256} rows)", flush=True)
+    g = torch.Generator().manual_seed(23)
+    x = torch.randn(n, cin, generator=g).to("cuda", dtype)
+    w = (0.05 * torch.randn(cout, cin, 3, 3, 3, generator=g)).to("cuda", dtype)
+    a = fvdb._autograd.weight_to_koi(w, False)   # [K, O, I]
+    bias = torch.randn(cout, generator=g).to("cuda", dtype)
+    perm = torch.randperm(n, generator=g).to("cuda", torch.int32)
+    for m in (pack._map_fwd(), pack._map_bwd()):
+        m = m.clone()
+        m[256:512] = -1   # rows 256..511 have no neighbour: one whole tile of the identity order
+        orders = {"identity": torch.arange(n, dtype=torch.int32, device="cuda"), "random": perm,
+                  "global": ext.row_order(m, n, 0), "windowed": ext.row_order(m, n, 2048)}
+        for b, relu in ((None, False), (bias, True)):
+            y = ext.implicit_gemm(x, a, m, n, None, b, relu)
+            assert bool(torch.isfinite(y).all())
+            for name, o in orders.items():
+                assert torch.equal(ext.implicit_gemm(x, a, m, n, o, b, relu), y), (name, b is not None, relu)
+    ref = oracle.sparse_conv(x.double().cpu(), w.double().cpu(), pack._map_fwd()[:n].cpu().numpy(), n, 1)
+    y = ext.implicit_gemm(x, a, pack._map_fwd(), n, orders["global"])
+    assert rel_err(y, ref) < REL_TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
 @pytest.mark.parametrize("cout", [256, 192])
 def test_direct_lds_wide_bit_exact(dtype, cout):
     """The direct-to-LDS contraction for 128 input channels (steps of one tap and one
@@ -561,10 +602,12 @@ def test_automatic_row_order_policy():
     assert p2._order_for(True, 64, bf) is None          # coarse rows: 1..8 taps each
     assert p2._order_for(False, 64, bf) is None         # fine rows (one tap each), narrow 16-bit
     from fvdb.sparse_conv_pack_info import wants_row_order
-    # large 128-channel 3x3x3 layers with 128k outputs stay in tree order (direct-to-LDS kernel)
-    assert not wants_row_order(27, 128, bf, 128, 300_000) and not wants_row_order(27, 128, bf, 256, 200_000)
-    assert not wants_row_order(27, 128, bf, 64, 300_000) and wants_row_order(27, 128, bf, 64, 200_000)
-    assert wants_row_order(27, 128, bf, 128, 100_000) and wants_row_order(27, 128, bf, 96, 300_000)
+    # 16-bit 3x3x3: 128 inputs, and 64 inputs with outputs in multiples of 128, take a row order
+    # (direct-to-LDS kernel with compacted steps); 64 -> 64 / 192 stay in tree order
+    assert wants_row_order(27, 128, bf, 128, 300_000) and wants_row_order(27, 128, bf, 64, 300_000)
+    assert wants_row_order(27, 64, bf, 128, 300_000) and wants_row_order(27, 64, bf, 256, 1_000)
+    assert not wants_row_order(27, 64, bf, 64, 300_000) and not wants_row_order(27, 64, bf, 192, 300_000)
+    assert not wants_row_order(8, 64, bf, 128, 300_000) and not wants_row_order(27, 32, bf, 128, 300_000)
     assert wants_row_order(27, 128, f32, 128, 300_000) and wants_row_order(8, 128, bf, 128, 300_000)
     o = p2._order_for(False, 64, f32)                  # fine rows in fp32: windowed order
     assert o is not None
