@@ -948,6 +948,192 @@ static bool launch_tile_dma_wide(const T* X, int64_t n_in, const T* A, const int
     return true;
 }
 
+// Narrow layers (16-bit, 32 output channels, CIN = 32 or 64 input channels, KT taps) with
+// the whole weight tensor resident in LDS.  At 32 channels a feature row is 64 bytes and a
+// map row 4*KT bytes, so these layers are bound by moving rows, not by the matrix cores;
+// with the weights resident nothing is staged per tap and the wavefronts never meet at a
+// barrier after the prologue.
+//  * Persistent workgroups of 16 wavefronts, one per CU (4 waves per SIMD).  The prologue
+//    copies the KT x 32 x CIN weights global -> LDS with the load unit (54 KB at 32 input
+//    channels, 108 KB at 64) and waits once.  Rows of 64 bytes are swizzled: 16-byte segment
+//    s of row r sits at position s ^ (((r >> 1) ^ (r >> 3)) & 3), so the MFMA operand reads
+//    of 8 (or 16) consecutive rows hit distinct banks.
+//  * Each wavefront owns tiles of 32 rows (2 groups of 16), a contiguous range per
+//    workgroup, and runs them independently.  A tile's map rows are read straight into
+//    registers (lane (g, r) holds taps [g*TPL, (g+1)*TPL) of its row), the next tile's while
+//    the current one runs, and each tap's neighbour index reaches the row's four lanes
+//    through a lane shuffle.
+//  * Gathers are raw buffer loads LX taps ahead; the lanes of a missing neighbour are
+//    switched off the load and keep zeros (the compiler's waits count the loads, as no
+//    direct-to-LDS copy is in flight; the texture data path then moves only rows that exist:
+//    level 0 of the U-Net, 32 -> 32: 0.36 -> 0.28 ms against loading zeros from beyond the
+//    buffer).  The MFMA order per output element is taps ascending, then 32-channel chunks,
+//    as in the other tile kernels, so the results are bit-identical to theirs.
+//  * The output tile goes through a wavefront-private LDS slice and leaves as whole rows.
+// Features must span fewer than 2^31 bytes (32-bit buffer offsets); callers check.
+template <typename T, int KT, int NCC, int LX>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void k_conv_resident(const T* __restrict__ X, int64_t n_in, const T* __restrict__ A, const int32_t* __restrict__ map,
+                     int64_t n_rows, int64_t n_wt, T* __restrict__ Y, const T* __restrict__ bias, int relu) {
+    static_assert(sizeof(T) == 2, "16-bit features");
+    using F = Frag<T>;
+    using V = typename F::vec;
+    constexpr int NW = 16, MI = 2, NI = 2, M = 32, CC = 32, CIN = CC * NCC;
+    constexpr int TPL = (KT + 3) / 4;                 // map entries per lane and row group
+    constexpr int SEGS = KT * NCC * 128;              // 16-byte segments of the weights
+    constexpr int W_BYTES = SEGS * 16;
+    constexpr int SLICE = NI * 16 * (M + 8);           // output slice (elements) per wavefront
+    static_assert(W_BYTES + NW * SLICE * (int)sizeof(T) <= 160 * 1024, "LDS budget");
+    extern __shared__ __attribute__((aligned(16))) unsigned char res_smem[];
+    T* const sW = reinterpret_cast<T*>(res_smem);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    auto swz = [](int r) { return ((r >> 1) ^ (r >> 3)) & 3; };
+
+    // weights -> LDS: segment p of block b = (tap, chunk) lands at b*128 + p; it holds
+    // row r = p/4, position p%4 = s ^ swz(r) of segment s (8 channels) of that row
+    for (int p0 = wave * 64; p0 < SEGS; p0 += NW * 64) {
+        const int p = p0 + lane;
+        if (p < SEGS) {
+            const int b = p >> 7, q = p & 127, r = q >> 2, s = (q & 3) ^ swz(r);
+            const int k = b / NCC, c = b % NCC;
+            lds_dma16(A + ((int64_t)k * M + r) * CIN + c * CC + s * 8, res_smem + (size_t)p0 * 16);
+        }
+    }
+    wait_vm_lds<0>();
+    __syncthreads();
+
+    const int64_t per = (n_wt + gridDim.x - 1) / gridDim.x;
+    const int64_t t_begin = (int64_t)blockIdx.x * per;
+    const int64_t t_end = t_begin + per < n_wt ? t_begin + per : n_wt;
+    const int bytes = (int)(n_in * CIN * (int64_t)sizeof(T));
+    const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, bytes, 0x00020000);
+    T* const slice = reinterpret_cast<T*>(res_smem + W_BYTES) + wave * SLICE;
+
+    // map entries of tile t: lane (g, r16) of row group j holds row t*32 + j*16 + r16, taps
+    // g*TPL + e (-1 beyond the rows or the taps)
+    auto load_map = [&](int64_t t, int32_t (&mr)[NI][TPL]) {
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+            const int64_t row = t * 32 + j * 16 + r16;
+            const bool ok = row < n_rows;
+            const int32_t* src = map + (ok ? row : 0) * KT;
+#pragma unroll
+            for (int e = 0; e < TPL; ++e) {
+                const int k = g * TPL + e;
+                const int32_t v = src[k < KT ? k : KT - 1];
+                mr[j][e] = (ok && k < KT) ? v : -1;
+            }
+        }
+    };
+    int32_t cur[NI][TPL], nxt[NI][TPL];
+    int64_t t = t_begin + wave;
+    if (t < t_end) load_map(t, nxt);
+    for (; t < t_end; t += NW) {
+#pragma unroll
+        for (int j = 0; j < NI; ++j)
+#pragma unroll
+            for (int e = 0; e < TPL; ++e) cur[j][e] = nxt[j][e];
+
+        floatx4 acc[MI][NI];
+#pragma unroll
+        for (int i = 0; i < MI; ++i)
+#pragma unroll
+            for (int j = 0; j < NI; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        V b[LX + 1][NI][NCC];
+        auto gather = [&](auto kc, V (&bb)[NI][NCC]) {
+            constexpr int k = decltype(kc)::value;
+#pragma unroll
+            for (int j = 0; j < NI; ++j) {
+                const int32_t src = __shfl(cur[j][k % TPL], (k / TPL) * 16 + r16);
+                const int off = src * (CIN * (int)sizeof(T)) + g * 16;
+#pragma unroll
+                for (int c = 0; c < NCC; ++c) bb[j][c] = zero_frag<T>();
+                if (src >= 0) {
+#pragma unroll
+                    for (int c = 0; c < NCC; ++c)
+                        bb[j][c] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(xr, off + c * 64, 0, 0));
+                }
+            }
+        };
+        static_for<0, (LX < KT ? LX : KT)>([&](auto kc) { gather(kc, b[decltype(kc)::value]); });
+        static_for<0, KT>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            if constexpr (k + LX < KT) gather(std::integral_constant<int, k + LX>{}, b[(k + LX) % (LX + 1)]);
+            // the next tile's map, halfway through the taps: loads complete in issue order, so
+            // issued at the start it would hold up the first gathers
+            if constexpr (k == KT / 2) {
+                if (t + NW < t_end) load_map(t + NW, nxt);
+            }
+#pragma unroll
+            for (int c = 0; c < NCC; ++c) {
+#pragma unroll
+                for (int i = 0; i < MI; ++i) {
+                    const int r = i * 16 + r16;
+                    const V a = *reinterpret_cast<const V*>(sW + ((k * NCC + c) * 128 + r * 4 + (g ^ swz(r))) * 8);
+#pragma unroll
+                    for (int j = 0; j < NI; ++j) mfma(a, b[k % (LX + 1)][j][c], acc[i][j]);
+                }
+            }
+        });
+        store_tile_slice<T, MI, NI>(Y, acc, slice, lane, t * 32, nullptr, n_rows, M, 0, bias, relu);
+    }
+}
+
+// Compute units of the current device (cached per device; the persistent kernels launch
+// one workgroup per CU).
+static inline int device_cu_count() {
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cus[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev] = n;
+    }
+    return cus[dev];
+}
+
+// Launches k_conv_resident for 16-bit layers of 32 output channels, 32 or 64 input
+// channels and 27 or 8 taps in tree order; false if the layer does not fit it.  U-Net
+// level 0 (3.2M rows), MI355X, against the register-staged tile kernel: 3x3x3 32 -> 32
+// 0.45 -> 0.28 ms, 64 -> 32 0.92 -> 0.68 ms, 2x2x2 transposed 64 -> 32 0.29 -> 0.14 ms.
+template <typename T>
+static bool launch_resident(const T* X, int64_t n_in, int cin, const T* A, const int32_t* map, int64_t n_rows, int K,
+                            int M, T* Y, const T* bias, int relu, hipStream_t s) {
+    if (M != 32 || (cin != 32 && cin != 64) || (K != 27 && K != 8) ||
+        n_in * cin * (int64_t)sizeof(T) > 0x7FFFFF00 || ((uintptr_t)Y % 16) != 0 || n_rows <= 0)
+        return false;
+    const int64_t n_wt = (n_rows + 31) / 32;
+    const int64_t want = (n_wt + 15) / 16;
+    const int n_cu = device_cu_count();
+    const unsigned grid = (unsigned)(want < n_cu ? want : n_cu);
+    auto go = [&](auto kt, auto ncc, auto lx) {
+        constexpr int KT = decltype(kt)::value, NCC = decltype(ncc)::value, LX = decltype(lx)::value;
+        const size_t lds = (size_t)KT * NCC * 128 * 16 + (size_t)16 * 2 * 16 * (32 + 8) * sizeof(T);
+        // more than 64 KB of dynamic LDS: raise the kernel's limit once
+        static const hipError_t attr = hipFuncSetAttribute(
+            reinterpret_cast<const void*>(&k_conv_resident<T, KT, NCC, LX>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)attr;
+        hipLaunchKernelGGL((k_conv_resident<T, KT, NCC, LX>), dim3(grid), dim3(1024), lds, s, X, n_in, A, map, n_rows,
+                           n_wt, Y, bias, relu);
+    };
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I4 = std::integral_constant<int, 4>;
+    using I8 = std::integral_constant<int, 8>;
+    using I27 = std::integral_constant<int, 27>;
+    if (K == 27) {
+        if (cin == 32) go(I27{}, I1{}, std::integral_constant<int, 3>{});
+        else go(I27{}, I2{}, I1{});
+    } else {
+        if (cin == 32) go(I8{}, I1{}, I4{});
+        else go(I8{}, I2{}, I2{});
+    }
+    return true;
+}
+
 // Direct-to-LDS contraction of 16-bit 3x3x3 layers over rows in a tap-mask order
 // (k_conv_tile_dma_perm, workgroups of 8 wavefronts = 256 rows): 64 or 128 input
 // channels, outputs in multiples of 64 channels (128-channel workgroups one step ahead
@@ -1351,6 +1537,8 @@ inline bool conv_tile_dispatch_t(const T* X, int64_t n_in, int cin, const T* A, 
         // rows through a tap-mask ordering: direct-to-LDS kernel with compacted steps
         if (perm != nullptr && launch_tile_dma_perm<T>(X, n_in, cin, A, map, perm, n_rows, K, M, Y, bias, relu, s))
             return true;
+        // narrow layers in tree order: weights resident in LDS
+        if (perm == nullptr && launch_resident<T>(X, n_in, cin, A, map, n_rows, K, M, Y, bias, relu, s)) return true;
     }
     switch (cin) {
     case 32: return try_tile<T, 32>(X, n_in, A, map, perm, n_rows, K, M, Y, bias, relu, s);

@@ -521,6 +521,46 @@ def test_direct_lds_ordered_rows_bit_exact(dtype, cin, cout):
 
 
 @pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
+@pytest.mark.parametrize("cin", [32, 64])
+@pytest.mark.parametrize("ks,st", [(3, 1), (2, 2)])
+def test_resident_weights_bit_exact(dtype, cin, ks, st):
+    """The narrow-layer contraction with its weights resident in LDS (16-bit, 32 output
+    channels, 32 or 64 input channels, 27 or 8 taps, rows in tree order; persistent
+    workgroups whose wavefronts loop over 32-row tiles) equals the tap-skipping kernel over an
+    identity row order bit for bit, with and without the fused bias + ReLU, for the forward
+    map (coarse rows for 2x2x2 stride 2) and the transposed map (one tap per fine row), on a
+    grid with more 32-row tiles than the launch has wavefronts and a ragged last tile; and the
+    fp64 oracle within the 16-bit bound."""
+    fvdb = _fvdb()
+    from fvdb._lib import ext
+    from fvdb.utils.synthetic import sphere_points, voxelize
+
+    grid = voxelize(sphere_points(seed=7, n_points=1_500_000, device="cuda"), 1 / 400)
+    pack, tgt = grid.sparse_conv_kernel_map(ks, st)
+    pack.build_implicit_gemm(training=True)
+    n_src, n_tgt = grid.total_voxels, tgt.total_voxels
+    g = torch.Generator().manual_seed(24)
+    w = (0.05 * torch.randn(32, cin, ks, ks, ks, generator=g)).to("cuda", dtype)
+    a = fvdb._autograd.weight_to_koi(w, False)   # [K, 32, cin]
+    bias = torch.randn(32, generator=g).to("cuda", dtype)
+    for m, n, n_in in ((pack._map_fwd(), n_tgt, n_src), (pack._map_bwd(), n_src, n_tgt)):
+        print(f"[resident] {dtype} {ks}^3 s{st} {cin}->32 rows={n} (last tile {n % 32} rows)", flush=True)
+        if ks == 3:
+            assert (n + 31) // 32 > 256 * 16, n
+        x = torch.randn(n_in, cin, generator=g).to("cuda", dtype)
+        ident = torch.arange(n, dtype=torch.int32, device="cuda")
+        for b, relu in ((None, False), (bias, True)):
+            y = ext.implicit_gemm(x, a, m, n, None, b, relu)
+            assert torch.equal(ext.implicit_gemm(x, a, m, n, ident, b, relu), y), (b is not None, relu)
+    small = voxelize(sphere_points(seed=8, n_points=60_000, device="cuda"), 1 / 96)
+    sp, st_grid = small.sparse_conv_kernel_map(ks, st)
+    x = torch.randn(small.total_voxels, cin, generator=g).to("cuda", dtype)
+    n = st_grid.total_voxels
+    ref = oracle.sparse_conv(x.double().cpu(), w.double().cpu(), sp._map_fwd()[:n].cpu().numpy(), n, st)
+    assert rel_err(ext.implicit_gemm(x, a, sp._map_fwd(), n), ref) < REL_TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
 @pytest.mark.parametrize("cout", [256, 192])
 def test_direct_lds_wide_bit_exact(dtype, cout):
     """The direct-to-LDS contraction for 128 input channels (steps of one tap and one
