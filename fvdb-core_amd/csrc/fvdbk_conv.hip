@@ -154,8 +154,15 @@ static int conv_mfma_dispatch(const T* X, int64_t n_in, int cin, const T* A, con
 // ---------------------------------------------------------------------------------
 // weight gradient: split-K over the neighbour-map pairs of each offset
 //
-// gw[k][c][o] = sum_p in[pi][c] * gout[po][o].  A workgroup takes one chunk of the
-// pairs of one offset and a (c, o) tile; rows of both operands are gathered into LDS
+// gw[k][c][o] = sum_p in[pi][c] * gout[po][o].  Offset k's pairs (ordered by output row)
+// are cut into rounds of WG_ROWS pairs and its n_k = chunk_start[k+1] - chunk_start[k]
+// partials take them interleaved: partial s sums rounds s, s + n_k, s + 2 n_k, ...  With
+// n_k proportional to the offset's pair count, the workgroups of every offset then sweep
+// the output rows together, so at any time they share a window of a few thousand rows
+// (the rows' next readers are the other offsets' workgroups, a few microseconds later)
+// instead of each reading its own stretch of the grid once: the 16-bit 128 x 128 layer of
+// 1.1M rows fetched 3.4 GB from HBM per launch with contiguous chunks (2% L2 hits).
+// A workgroup takes one partial of one offset and a (c, o) tile; rows of both operands are gathered into LDS
 // (16-byte loads), then read back as MFMA fragments whose reduction index is the
 // pair: for 16-bit types lane group g, element e takes pair row  step*CH + 8g + e  of
 // the stage through LDS transpose reads (two 8-byte reads per fragment); for fp32
@@ -178,17 +185,19 @@ __device__ __host__ constexpr int wg_swizzle(int r) {
                        : 0;
 }
 
-// Software pipeline: each thread owns fixed (pair row, channel group)
-// slots of the staged tiles.  While round r is contracted from LDS, the rows of round
-// r+1 are in flight into registers (their pair indices were loaded one round earlier)
-// and the pair indices of round r+2 are loaded; the rows are written to LDS at the
-// start of the next round.  Every global latency is thus covered by one round of MFMA
-// work instead of being paid in full each round.
-template <typename T, int MI, int NI>
+// Software pipeline: each thread owns fixed (pair row, channel group) slots of the
+// staged tiles.  While round r is contracted from LDS, the rows of round r+1 are in flight
+// into registers (their pair indices were loaded one round earlier) and the pair indices
+// of round r+2 are loaded; a round's rows are written to LDS at its start.  The 16-bit
+// gathers come mostly from the last-level cache, several microseconds away under load,
+// so the rate is set by the bytes each CU keeps in flight: two workgroups of 32 KB rounds
+// per CU (a third register set for a second round ahead costs half the occupancy and
+// measured slower).
+template <typename T, int MI, int NI, bool VEC>
 __global__ __launch_bounds__(256) void k_wgrad_pipe(const T* __restrict__ X, int cin, const T* __restrict__ G,
                                                     int cout, const int32_t* __restrict__ pairs,
                                                     const int64_t* __restrict__ k_offsets,
-                                                    const int32_t* __restrict__ chunk_start, int K, int chunk,
+                                                    const int32_t* __restrict__ chunk_start, int K,
                                                     int transposed, float* __restrict__ ws) {
     using F = Frag<T>;
     using V = typename F::vec;
@@ -213,8 +222,10 @@ __global__ __launch_bounds__(256) void k_wgrad_pipe(const T* __restrict__ X, int
         else hi = mid;
     }
     const int k = lo;
-    const int64_t p0 = k_offsets[k] + (int64_t)(id - chunk_start[k]) * chunk;
-    const int64_t p1 = min(p0 + chunk, k_offsets[k + 1]);
+    // partial s of offset k takes rounds s, s + n_k, ...: round j starts at pair p0 + j * stride
+    const int64_t stride = (int64_t)(chunk_start[k + 1] - chunk_start[k]) * WG_ROWS;
+    const int64_t p0 = k_offsets[k] + (int64_t)(id - chunk_start[k]) * WG_ROWS;
+    const int64_t p1 = k_offsets[k + 1];
 
     const int n_ct = (cin + TM - 1) / TM;
     const int c0 = (blockIdx.y % n_ct) * TM;
@@ -223,8 +234,6 @@ __global__ __launch_bounds__(256) void k_wgrad_pipe(const T* __restrict__ X, int
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, r16 = lane & 15;
     const int wm = (wave & 1) * MI * 16, wn = (wave >> 1) * NI * 16;
-    const bool vec_x = (cin % F::EPL) == 0;
-    const bool vec_g = (cout % F::EPL) == 0;
 
     floatx4 acc[MI][NI];
 #pragma unroll
@@ -232,69 +241,66 @@ __global__ __launch_bounds__(256) void k_wgrad_pipe(const T* __restrict__ X, int
 #pragma unroll
         for (int j = 0; j < NI; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
+    // VEC (channel counts multiples of the 16-byte fragment, 16-byte aligned rows): every
+    // round issues the same loads, an entry past the partial's last pair (or past the
+    // channels) loading row 0 and being zeroed when stored, so there are no branches around
+    // loads and the wait before a round's stores covers exactly its rows.
     int32_t xi[XPT], gi[GPT];  // pair indices of the round after next
     V xv[XPT], gv[GPT];        // rows of the next round
-    auto load_idx = [&](int64_t p) {
+    uint32_t vmask;            // bit q (X) / XPT + q (G): the entry is a real pair row
+    auto load_idx = [&](int64_t p, int32_t (&xx)[XPT], int32_t (&gg)[GPT]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int q = 0; q < XPT; ++q) {
+            const int rr = (tid + 256 * q) / XSEG;
+            const int64_t pp = p + rr < p1 ? p + rr : p1 - 1;
+            xx[q] = pairs[2 * pp + ci];
+        }
+#pragma unroll
+        for (int q = 0; q < GPT; ++q) {
+            const int rr = (tid + 256 * q) / GSEG;
+            const int64_t pp = p + rr < p1 ? p + rr : p1 - 1;
+            gg[q] = pairs[2 * pp + co];
+        }
+    };
+    auto load_rows = [&](int64_t p, const int32_t (&xx)[XPT], const int32_t (&gg)[GPT], V (&xr)[XPT],
+                         V (&gr)[GPT], uint32_t& m) __attribute__((always_inline)) {
+        m = 0u;
 #pragma unroll
         for (int q = 0; q < XPT; ++q) {
             const int e = tid + 256 * q, rr = e / XSEG;
-            xi[q] = (e < WG_ROWS * XSEG && p + rr < p1) ? pairs[2 * (p + rr) + ci] : -1;
+            const int c = c0 + (e % XSEG) * F::EPL;
+            const bool ok = e < WG_ROWS * XSEG && p + rr < p1 && c < cin;
+            m |= (uint32_t)ok << q;
+            if constexpr (VEC) xr[q] = *reinterpret_cast<const V*>(X + (ok ? (int64_t)xx[q] * cin + c : 0));
+            else xr[q] = ok ? load_frag<T, false>(X + (int64_t)xx[q] * cin, c, cin) : zero_frag<T>();
         }
 #pragma unroll
         for (int q = 0; q < GPT; ++q) {
             const int e = tid + 256 * q, rr = e / GSEG;
-            gi[q] = (e < WG_ROWS * GSEG && p + rr < p1) ? pairs[2 * (p + rr) + co] : -1;
-        }
-    };
-    auto load_rows = [&]() {
-#pragma unroll
-        for (int q = 0; q < XPT; ++q) {
-            const int e = tid + 256 * q;
-            const int c = c0 + (e % XSEG) * F::EPL;
-            xv[q] = zero_frag<T>();
-            if (xi[q] >= 0) {
-                const T* src = X + (int64_t)xi[q] * cin;
-                xv[q] = vec_x ? load_frag<T, true>(src, c, cin) : load_frag<T, false>(src, c, cin);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < GPT; ++q) {
-            const int e = tid + 256 * q;
             const int o = o0 + (e % GSEG) * F::EPL;
-            gv[q] = zero_frag<T>();
-            if (gi[q] >= 0) {
-                const T* src = G + (int64_t)gi[q] * cout;
-                gv[q] = vec_g ? load_frag<T, true>(src, o, cout) : load_frag<T, false>(src, o, cout);
-            }
+            const bool ok = e < WG_ROWS * GSEG && p + rr < p1 && o < cout;
+            m |= (uint32_t)ok << (XPT + q);
+            if constexpr (VEC) gr[q] = *reinterpret_cast<const V*>(G + (ok ? (int64_t)gg[q] * cout + o : 0));
+            else gr[q] = ok ? load_frag<T, false>(G + (int64_t)gg[q] * cout, o, cout) : zero_frag<T>();
         }
     };
-    auto store_rows = [&]() {
+    auto store_rows = [&](const V (&xr)[XPT], const V (&gr)[GPT], uint32_t m) __attribute__((always_inline)) {
+        const V z = zero_frag<T>();
 #pragma unroll
         for (int q = 0; q < XPT; ++q) {
             const int e = tid + 256 * q, r = e / XSEG;
             const int seg = TR ? (e % XSEG) ^ wg_swizzle<XSEG>(r) : e % XSEG;
-            if (e < WG_ROWS * XSEG) *reinterpret_cast<V*>(&sX[r * XP + seg * F::EPL]) = xv[q];
+            if (e < WG_ROWS * XSEG) *reinterpret_cast<V*>(&sX[r * XP + seg * F::EPL]) = ((m >> q) & 1u) ? xr[q] : z;
         }
 #pragma unroll
         for (int q = 0; q < GPT; ++q) {
             const int e = tid + 256 * q, r = e / GSEG;
             const int seg = TR ? (e % GSEG) ^ wg_swizzle<GSEG>(r) : e % GSEG;
-            if (e < WG_ROWS * GSEG) *reinterpret_cast<V*>(&sG[r * GP + seg * F::EPL]) = gv[q];
+            if (e < WG_ROWS * GSEG)
+                *reinterpret_cast<V*>(&sG[r * GP + seg * F::EPL]) = ((m >> (XPT + q)) & 1u) ? gr[q] : z;
         }
     };
-
-    if (p0 < p1) {
-        load_idx(p0);
-        load_rows();
-        load_idx(p0 + WG_ROWS);
-    }
-    for (int64_t p = p0; p < p1; p += WG_ROWS) {
-        store_rows();
-        __syncthreads();
-        if (p + WG_ROWS < p1) {
-            load_rows();                  // rows of round p + WG_ROWS
-            load_idx(p + 2 * WG_ROWS);    // indices of the round after
-        }
+    auto contract = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int st = 0; st < WG_ROWS / F::CH; ++st) {
             V a[MI], b[NI];
@@ -330,6 +336,19 @@ __global__ __launch_bounds__(256) void k_wgrad_pipe(const T* __restrict__ X, int
 #pragma unroll
                 for (int j = 0; j < NI; ++j) mfma(a[i], b[j], acc[i][j]);
         }
+    };
+    const int64_t n_rounds = p0 < p1 ? (p1 - p0 + stride - 1) / stride : 0;
+    if (n_rounds > 0) {
+        load_idx(p0, xi, gi);
+        load_rows(p0, xi, gi, xv, gv, vmask);
+        load_idx(p0 + stride, xi, gi);
+    }
+    for (int64_t j = 0; j < n_rounds; ++j) {
+        store_rows(xv, gv, vmask);
+        __syncthreads();
+        load_rows(p0 + (j + 1) * stride, xi, gi, xv, gv, vmask);  // rows of round j + 1
+        load_idx(p0 + (j + 2) * stride, xi, gi);                  // indices of round j + 2
+        contract();
         __syncthreads();
     }
 
@@ -350,7 +369,7 @@ __global__ __launch_bounds__(256) void k_wgrad_pipe(const T* __restrict__ X, int
 template <typename T, typename Acc>
 __global__ void k_wgrad_generic(const T* __restrict__ X, int cin, const T* __restrict__ G, int cout,
                                 const int32_t* __restrict__ pairs, const int64_t* __restrict__ k_offsets,
-                                const int32_t* __restrict__ chunk_start, int K, int chunk, int64_t total_chunks,
+                                const int32_t* __restrict__ chunk_start, int K, int64_t total_chunks,
                                 int transposed, Acc* __restrict__ ws) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t per = (int64_t)cin * cout;
@@ -363,12 +382,14 @@ __global__ void k_wgrad_generic(const T* __restrict__ X, int cin, const T* __res
         if (chunk_start[mid] <= id) lo = mid;
         else hi = mid;
     }
-    const int64_t p0 = k_offsets[lo] + (int64_t)(id - chunk_start[lo]) * chunk;
-    const int64_t p1 = min(p0 + chunk, k_offsets[lo + 1]);
+    // the same interleaved rounds as k_wgrad_pipe
+    const int64_t stride = (int64_t)(chunk_start[lo + 1] - chunk_start[lo]) * WG_ROWS;
+    const int64_t p1 = k_offsets[lo + 1];
     const int ci = transposed ? 1 : 0, co = 1 - ci;
     Acc acc = 0;
-    for (int64_t p = p0; p < p1; ++p)
-        acc += (Acc)X[(int64_t)pairs[2 * p + ci] * cin + c] * (Acc)G[(int64_t)pairs[2 * p + co] * cout + o];
+    for (int64_t r = k_offsets[lo] + (int64_t)(id - chunk_start[lo]) * WG_ROWS; r < p1; r += stride)
+        for (int64_t p = r; p < r + WG_ROWS && p < p1; ++p)
+            acc += (Acc)X[(int64_t)pairs[2 * p + ci] * cin + c] * (Acc)G[(int64_t)pairs[2 * p + co] * cout + o];
     ws[idx] = acc;
 }
 
@@ -402,12 +423,18 @@ __global__ void k_fill_zero_u32(uint32_t* __restrict__ p, int64_t n) {
 
 template <typename T, int MI, int NI>
 static void launch_wgrad_mfma(const T* X, int cin, const T* G, int cout, const int32_t* pairs, const int64_t* koff,
-                              const int32_t* cstart, int K, int chunk, int64_t total_chunks, int transposed,
-                              float* ws, hipStream_t s) {
+                              const int32_t* cstart, int K, int64_t total_chunks, int transposed, float* ws,
+                              hipStream_t s) {
     constexpr int TM = 2 * MI * 16, TN = 2 * NI * 16;
-    const unsigned tiles = (unsigned)(((cin + TM - 1) / TM) * ((cout + TN - 1) / TN));
-    hipLaunchKernelGGL((k_wgrad_pipe<T, MI, NI>), dim3((unsigned)total_chunks, tiles), dim3(256), 0, s, X, cin, G,
-                       cout, pairs, koff, cstart, K, chunk, transposed, ws);
+    const dim3 grid((unsigned)total_chunks, (unsigned)(((cin + TM - 1) / TM) * ((cout + TN - 1) / TN)));
+    const bool vec = cin % Frag<T>::EPL == 0 && cout % Frag<T>::EPL == 0 && (uintptr_t)X % 16 == 0 &&
+                     (uintptr_t)G % 16 == 0;
+    if (vec)
+        hipLaunchKernelGGL((k_wgrad_pipe<T, MI, NI, true>), grid, dim3(256), 0, s, X, cin, G, cout, pairs, koff, cstart,
+                           K, transposed, ws);
+    else
+        hipLaunchKernelGGL((k_wgrad_pipe<T, MI, NI, false>), grid, dim3(256), 0, s, X, cin, G, cout, pairs, koff,
+                           cstart, K, transposed, ws);
 }
 
 template <typename T>
@@ -417,7 +444,7 @@ static int wgrad_mfma_dispatch(const T* X, int cin, const T* G, int cout, const 
     const int mi = cin <= 32 ? 1 : (cin <= 64 ? 2 : 4);
     const int ni = cout <= 32 ? 1 : (cout <= 64 ? 2 : 4);
 #define FVDBK_WG(MI_, NI_) \
-    launch_wgrad_mfma<T, MI_, NI_>(X, cin, G, cout, pairs, koff, cstart, K, chunk, total_chunks, transposed, ws, s)
+    launch_wgrad_mfma<T, MI_, NI_>(X, cin, G, cout, pairs, koff, cstart, K, total_chunks, transposed, ws, s)
     if (mi == 1 && ni == 1) FVDBK_WG(1, 1);
     else if (mi == 1 && ni == 2) FVDBK_WG(1, 2);
     else if (mi == 1 && ni == 4) FVDBK_WG(1, 4);
@@ -588,7 +615,7 @@ int fvdbk_conv_wgrad(fvdbk_dtype dtype, const void* in, int32_t c_in, const void
         const int64_t m = total_chunks * c_in * c_out;
         hipLaunchKernelGGL((k_wgrad_generic<double, double>), dim3(grid_1d(m, 256)), dim3(256), 0, stream,
                            (const double*)in, c_in, (const double*)gout, c_out, pairs, k_offsets, chunk_start, K,
-                           chunk, total_chunks, transposed, (double*)workspace);
+                           total_chunks, transposed, (double*)workspace);
         FVDBK_LAUNCH_CHECK("k_wgrad_generic");
         hipLaunchKernelGGL(k_wgrad_reduce<double>, dim3(grid_1d(n, 256)), dim3(256), 0, stream,
                            (const double*)workspace, chunk_start, K, c_in, c_out, (double*)gw);

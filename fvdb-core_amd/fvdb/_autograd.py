@@ -9,8 +9,9 @@ mapped back with the inverse permutations (``:170-187``).
 
 * forward: implicit GEMM over the kernel map (rows = output voxels);
 * dgrad:   the same kernel over the opposite map with the un-transposed weights;
-* wgrad:   split-K MFMA reduction over the neighbour-map pairs with an fp32
-  Neumaier-compensated sum of the chunk partials.
+* wgrad:   split-K MFMA reduction over the neighbour-map pairs (each offset's 64-pair
+  rounds interleaved over its partials) with an fp32 Neumaier-compensated sum of the
+  partials.
 * row order: rows grouped by tap mask (the pack's ``_order_for``) so that a tile skips
   the taps none of its rows uses; the result does not depend on the order.
 * epilogue: an optional bias and ReLU applied to the fp32 accumulator before the
@@ -48,14 +49,16 @@ def _wgrad_tile(c: int) -> int:
 
 def wgrad_chunk(n_pairs: int, dtype: torch.dtype = torch.float32, cin: int = 128, cout: int = 128,
                 n_cu: int = 0) -> int:
-    """Pairs per split-K chunk, a multiple of 64 and at least 256.
+    """Pairs per split-K chunk, a multiple of 64 and at least 256: offset k gets
+    ceil(pairs of k / chunk) partials, which take its 64-pair rounds interleaved.
 
-    fp32 (MFMA-bound) runs best with ~4096 chunks per launch.  16-bit inputs are bound by
-    their gathers, and run best when the launch is one wave of workgroups that all stay
-    resident: 128 x 128 channel tiles hold two workgroups per CU, and a launch just above that
-    pays a second, mostly idle wave (planes + spheres, 6.7M pairs, 128 -> 128 on MI355X: 462
-    chunks 0.58 ms, 525 chunks 0.81 ms).  Narrower tiles measured best at 3 (64-channel) and 4
-    (32-channel) workgroups per CU.  Workgroups per chunk = channel tiles of the layer."""
+    fp32 (MFMA-bound) runs best with ~2048 partials per launch (planes + spheres, 6.7M pairs,
+    128 -> 128 on MI355X: 2.40 ms against 2.52 ms at 473 and 2.60 ms at 4040).  16-bit inputs
+    are bound by their gathers, and run best when the launch is one wave of workgroups that all
+    stay resident: 128 x 128 channel tiles hold two workgroups per CU, and a launch just above
+    that pays a second, mostly idle wave (same layer: 473 partials 0.51 ms, 525 partials
+    0.67 ms).  Narrower tiles measured best at 3 (64-channel) and 4 (32-channel) workgroups
+    per CU.  Workgroups per partial = channel tiles of the layer."""
     if dtype in (torch.bfloat16, torch.float16):
         if n_cu <= 0:
             n_cu = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count \
@@ -65,7 +68,7 @@ def wgrad_chunk(n_pairs: int, dtype: torch.dtype = torch.float32, cin: int = 128
         per_cu = 1.75 if (tm, tn) == (128, 128) else (3.0 if max(tm, tn) >= 64 else 4.0)
         target = max(1, int(per_cu * n_cu / tiles))
     else:
-        target = 4096
+        target = 2048
     c = -(-max(n_pairs, 1) // target)
     c = -(-c // 64) * 64
     return int(max(256, c))

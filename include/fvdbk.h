@@ -298,12 +298,14 @@ int fvdbk_kmap_row_order_windowed(const int32_t* map, int64_t n_rows, int32_t K,
  *   gw[k][c][o] = sum over pairs p of offset k:  in[pairs[p][ci]][c] * gout[pairs[p][co]][o]
  *
  * with (ci, co) = (0, 1), or (1, 0) when transposed.  Pairs are the neighbour map
- * (ordered by k).  Split-K: the pairs of each offset are cut into chunks of
- * `chunk` pairs (chunk_start[k] = first chunk of offset k, chunk_start[K] = total
- * chunks, a device int32 array); each chunk's fp32 partial product goes to
- * workspace [total_chunks][c_in][c_out] and a second pass adds the partials of each
- * offset with Neumaier-compensated fp32 summation (fp64 plain sums for FVDBK_F64),
- * in chunk order, so the result is deterministic.  k_offsets [K+1] (device int64)
+ * (ordered by k, then output row).  Split-K: offset k has n_k = chunk_start[k+1] -
+ * chunk_start[k] partials (chunk_start[k] = first partial of offset k, chunk_start[K] =
+ * total_chunks, a device int32 array; the callers take n_k = ceil(pairs of k / chunk)).
+ * The offset's pairs are cut into rounds of 64 and partial s sums rounds s, s + n_k,
+ * s + 2 n_k, ..., so the partials of all offsets sweep the output rows together.  Each
+ * partial's fp32 product goes to workspace [total_chunks][c_in][c_out] and a second pass
+ * adds the partials of each offset with Neumaier-compensated fp32 summation (fp64 plain
+ * sums for FVDBK_F64), in partial order, so the result is deterministic.  k_offsets [K+1] (device int64)
  * are the pair offsets of each k.  gw is fp32 ([K][c_in][c_out]) for the 16/32-bit
  * dtypes and fp64 for FVDBK_F64.
  * ------------------------------------------------------------------------- */
