@@ -214,6 +214,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=40,
                     help="requests of the short serving stream reported in the default line (0: none)")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: Poisson arrival rate (req/s); 0 = auto")
+    ap.add_argument("--serve-ingest-workers", type=int, default=1, help="serve: ingest host threads (0 = inline)")
+    ap.add_argument("--serve-streams", type=int, default=3, help="serve: compute streams (= batches in flight)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -327,7 +329,8 @@ def main() -> int:
     if args.serve_requests > 0:
         # the metric's serving half: a short Poisson stream through the U-Net workload's network
         log(f"[bench] serving stream of {args.serve_requests} requests")
-        sv = serve_stream(torch, fvdb, dev, args.serve_requests)
+        sv = serve_stream(torch, fvdb, dev, args.serve_requests, ingest_workers=args.serve_ingest_workers,
+                          streams=args.serve_streams)
         cb = sv["bf16_continuous_batching"]
         serve = {"workload": f"{args.serve_requests} single-grid requests (log-uniform 10k-2M voxels, Poisson "
                              f"arrivals at 70% of the closed-batch service rate), 4-level U-Net 32/64/128/256",
@@ -651,9 +654,11 @@ def run_unet(torch, fvdb, dev, args):
                                "fp32": rel(outs["fp32"], outs["fp64"])}}
 
 
-def serve_stream(torch, fvdb, dev, nreq, rate=0.0, seed=42, budget_factor=1.2):
+def serve_stream(torch, fvdb, dev, nreq, rate=0.0, seed=42, budget_factor=1.2, ingest_workers=1, streams=3):
     """Poisson stream of single grids (log-uniform 10k-2M voxels) through the 4-level U-Net in bf16
     with continuous batching, against one-at-a-time loops (bf16 and fp32) over the same arrivals.
+    Requests are ingested on `ingest_workers` host threads (0: inline in the loop) and batches run
+    on `streams` compute streams (at most that many batches in flight).
 
     The HBM budget is `budget_factor` times the largest request's admission bound, so admission
     binds; the device memory the stream allocates beyond what was live before it is measured
@@ -695,23 +700,25 @@ def serve_stream(torch, fvdb, dev, nreq, rate=0.0, seed=42, budget_factor=1.2):
         return {"p50_ms": 1e3 * lat[(len(lat) - 1) // 2], "p99_ms": 1e3 * lat[k99],
                 "throughput_voxels_per_s": sum(counts) / span, "throughput_req_per_s": nreq / span, "span_s": span}
 
-    ex = UNetExecutor(model, dtype=torch.bfloat16, host_results=True)
+    ex = UNetExecutor(model, dtype=torch.bfloat16, host_results=True, compute_streams=streams)
     budget = int(budget_factor * max(ex.estimate_bytes(Request(rid=i, ijk=t)) for i, t in enumerate(reqs_ijk)))
-    warm = ContinuousBatcher(ex, hbm_budget_bytes=budget)
+    warm = ContinuousBatcher(ex, hbm_budget_bytes=budget, max_inflight=streams, ingest_workers=ingest_workers)
     warm.serve([Request(rid=i, ijk=reqs_ijk[i], seed=100 + i) for i in range(min(8, nreq))])
     # the set-up's objects out of the cyclic collector's full scans
     gc.collect()
     gc.freeze()
     t0 = time.perf_counter()
-    closed = ContinuousBatcher(ex, hbm_budget_bytes=budget)
+    closed = ContinuousBatcher(ex, hbm_budget_bytes=budget, max_inflight=streams, ingest_workers=ingest_workers)
     closed.serve([Request(rid=i, ijk=reqs_ijk[i], seed=100 + i) for i in range(nreq)])
     torch.cuda.synchronize()
     service_rate = nreq / (time.perf_counter() - t0)
+    closed_batch = statistics.mean(closed.batches)
     rate = rate if rate > 0 else 0.7 * service_rate
     gexp = torch.Generator().manual_seed(7)
     gaps = torch.empty(nreq, dtype=torch.float64).exponential_(rate, generator=gexp)
     arrivals = torch.cumsum(gaps, 0).tolist()
-    batcher = ContinuousBatcher(ex, hbm_budget_bytes=budget, max_batch_voxels=6_000_000, max_inflight=2)
+    batcher = ContinuousBatcher(ex, hbm_budget_bytes=budget, max_batch_voxels=6_000_000, max_inflight=streams,
+                                ingest_workers=ingest_workers)
     waits = [0]
     admit = batcher._admit
 
@@ -730,16 +737,24 @@ def serve_stream(torch, fvdb, dev, nreq, rate=0.0, seed=42, budget_factor=1.2):
     ok = [r for r in done if r.status == "done"]
     stats = latency_stats(done)
     slow = sorted(ok, key=lambda r: -r.latency)[:3]
+    for r in sorted(ok, key=lambda r: r.arrival):
+        log(f"[bench] req #{r.rid:3d} {counts[r.rid]:8d} vox  arr {r.arrival * 1e3:7.2f}  admit +{(r.t_admit - r.arrival) * 1e3:6.2f}"
+            f"  prep +{(r.t_prepared - r.arrival) * 1e3:6.2f}  launch +{(r.t_launch - r.arrival) * 1e3:6.2f}"
+            f"  done +{r.latency * 1e3:6.2f} ms")
     log("[bench] slowest requests: " + ", ".join(f"#{r.rid} {counts[r.rid]} voxels arrived {r.arrival * 1e3:.1f} ms "
                                                  f"admitted +{(r.t_admit - r.arrival) * 1e3:.1f} latency "
                                                  f"{r.latency * 1e3:.1f} ms" for r in slow))
     serve = {"requests": nreq, "completed": len(ok), "rate_req_per_s": rate, "service_rate_closed": service_rate,
+             "mean_batch_closed": closed_batch,
              **stats, "throughput_voxels_per_s": sum(counts[r.rid] for r in ok) / span,
              "throughput_req_per_s": len(ok) / span, "span_s": span,
              "mean_batch": statistics.mean(batcher.batches), "max_batch": max(batcher.batches),
              "hbm_budget_GB": budget / 1e9, "peak_reserved_by_admission_GB": batcher.peak_reserved / 1e9,
              "measured_peak_allocated_GB": peak / 1e9, "measured_peak_within_budget": bool(peak <= budget),
              "admission_steps_with_requests_waiting": waits[0]}
+    for bt in (warm, closed, batcher):
+        bt.close()
+    serve["ingest_workers"], serve["compute_streams"] = ingest_workers, streams
     log(f"[bench] serve bf16 continuous batching: {serve}")
     base_bf16 = one_at_a_time(arrivals, torch.bfloat16)
     log(f"[bench] serve bf16 one at a time: {base_bf16}")
@@ -751,7 +766,8 @@ def serve_stream(torch, fvdb, dev, nreq, rate=0.0, seed=42, budget_factor=1.2):
 
 def run_serve(torch, fvdb, dev, args):
     """The serving workload: a Poisson stream of `--requests` single grids (default 120)."""
-    rec = serve_stream(torch, fvdb, dev, args.requests, args.rate)
+    rec = serve_stream(torch, fvdb, dev, args.requests, args.rate, ingest_workers=args.serve_ingest_workers,
+                       streams=args.serve_streams)
     return {"workload": "serve: Poisson stream, single grids log-uniform 10k-2M voxels, 4-level U-Net", **rec}
 
 

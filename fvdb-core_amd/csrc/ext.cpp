@@ -593,23 +593,27 @@ std::string version() { return fvdbk_version(); }
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.doc() = "fvdb-core_amd tensor layer over the fvdbk C ABI";
+    // Every entry point runs without the GIL (arguments are converted before, results after),
+    // so a host thread blocked in one of them -- a grid build waits for its sizes -- does not
+    // stall the other Python threads (the serving loop's ingest worker).
+    const auto nogil = py::call_guard<py::gil_scoped_release>();
     m.def("version", &version);
-    m.def("build_grid", &build_grid);
-    m.def("coarsen_grid", &coarsen_grid);
-    m.def("ijk_to_index", &ijk_to_index);
-    m.def("points_to_ijk", &points_to_ijk);
-    m.def("conv_grid_candidates", &conv_grid_candidates);
-    m.def("kernel_map", &kernel_map);
-    m.def("neighbor_map", &neighbor_map);
-    m.def("window_map", &window_map);
-    m.def("pool_max", &pool_max);
-    m.def("pool_avg", &pool_avg);
-    m.def("implicit_gemm", &implicit_gemm, pybind11::arg("in"), pybind11::arg("a"), pybind11::arg("map"),
+    m.def("build_grid", &build_grid, nogil);
+    m.def("coarsen_grid", &coarsen_grid, nogil);
+    m.def("ijk_to_index", &ijk_to_index, nogil);
+    m.def("points_to_ijk", &points_to_ijk, nogil);
+    m.def("conv_grid_candidates", &conv_grid_candidates, nogil);
+    m.def("kernel_map", &kernel_map, nogil);
+    m.def("neighbor_map", &neighbor_map, nogil);
+    m.def("window_map", &window_map, nogil);
+    m.def("pool_max", &pool_max, nogil);
+    m.def("pool_avg", &pool_avg, nogil);
+    m.def("implicit_gemm", &implicit_gemm, nogil, pybind11::arg("in"), pybind11::arg("a"), pybind11::arg("map"),
           pybind11::arg("n_rows"), pybind11::arg("order") = pybind11::none(), pybind11::arg("bias") = pybind11::none(),
           pybind11::arg("relu") = false);
-    m.def("row_order", &row_order, py::arg("map"), py::arg("n_rows"), py::arg("window") = 0);
-    m.def("wgrad", &wgrad);
-    m.def("wgrad_tabled", &wgrad_tabled);
-    m.def("group_norm_forward", &group_norm_forward);
-    m.def("group_norm_backward", &group_norm_backward);
+    m.def("row_order", &row_order, nogil, py::arg("map"), py::arg("n_rows"), py::arg("window") = 0);
+    m.def("wgrad", &wgrad, nogil);
+    m.def("wgrad_tabled", &wgrad_tabled, nogil);
+    m.def("group_norm_forward", &group_norm_forward, nogil);
+    m.def("group_norm_backward", &group_norm_backward, nogil);
 }

@@ -2,9 +2,10 @@
 
 Requests are single grids (voxel coordinates plus features) that arrive over time.
 
-* **Ingest** (a separate HIP stream): each admitted request's grid index and all the kernel maps its network
-  needs (one 3x3x3 map per level, the stride-2 down map and its transpose between levels) are built once with
-  the gfx950 kernels and stay resident on the device with the request.
+* **Ingest** (a separate HIP stream, optionally on a host thread of its own): each admitted request's grid index
+  and all the kernel maps its network needs (one 3x3x3 map per level, the stride-2 down map and its transpose
+  between levels) are built once with the gfx950 kernels and stay resident on the device with the request. A
+  grid build reads two sizes back to the host; on the ingest thread that wait does not hold up the loop.
 * **Admission**: a request is admitted (FIFO) when an upper bound of its device memory (maps, features,
   activations, output, ingest buffers) fits in the remaining HBM budget. Before ingest only the input voxel count
   is known, so the bound takes every level as large as the input; once the request is ingested its level sizes
@@ -12,8 +13,9 @@ Requests are single grids (voxel coordinates plus features) that arrive over tim
   when the request's batch completes or fails.
 * **Batching**: ready requests are packed (FIFO, up to a voxel cap) into one jagged launch per layer. The
   batch's kernel maps are the per-request maps concatenated with their indices shifted by the request's row
-  offset at that level, so no grid is rebuilt per batch. Launches go to the compute stream while the next
-  requests are ingested on the ingest stream, so their work overlaps on the device.
+  offset at that level, so no grid is rebuilt per batch. Each batch goes to a compute stream whose previous
+  batch is done (one stream per batch in flight), so a small batch runs beside a large one instead of behind
+  it, while the next requests are ingested on the ingest stream.
 * **Results**: every output row of the contraction is computed by one lane in a fixed order, independent of
   the other rows of the launch (see ``fvdbk_conv_implicit_gemm`` in ``include/fvdbk.h``), and bias/ReLU/concat
   are per row. A request's output is therefore bit-identical alone and in any mix.
@@ -26,6 +28,7 @@ from __future__ import annotations
 
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import Any, Callable, Dict, List, Optional, Sequence
 
@@ -43,11 +46,13 @@ class Request:
     features: Any = None          # [N, C] or None (then generated from `seed` on the device)
     seed: int = 0
     arrival: float = 0.0          # seconds on the batcher's clock
-    status: str = "new"           # new -> pending -> prepared -> running -> done | failed
+    status: str = "new"           # new -> pending -> [ingesting ->] prepared -> running -> done | failed
     error: Optional[BaseException] = None
     result: Any = None
     reserved: int = 0
     t_admit: float = math.nan
+    t_prepared: float = math.nan  # ingest finished (host side)
+    t_launch: float = math.nan
     t_done: float = math.nan
     state: Any = None             # executor-private (topology, device tensors)
 
@@ -60,7 +65,7 @@ class ContinuousBatcher:
     """Admission by memory budget, FIFO batching, bounded batches in flight."""
 
     def __init__(self, executor, hbm_budget_bytes: int, max_batch_voxels: int = 4_000_000, max_inflight: int = 2,
-                 clock: Callable[[], float] = time.perf_counter):
+                 clock: Callable[[], float] = time.perf_counter, ingest_workers: int = 0):
         self.ex = executor
         self.budget = int(hbm_budget_bytes)
         self.max_batch_voxels = int(max_batch_voxels)
@@ -69,7 +74,13 @@ class ContinuousBatcher:
         self.t0 = clock()
         self.reserved = 0
         self.peak_reserved = 0
+        # ingest_workers > 0: admitted requests are ingested (``prepare``) on host threads, so the
+        # loop keeps completing and launching batches while an ingest waits for the device
+        # (a grid build reads its sizes back); 0: ingest inline in the loop
+        self._pool = (ThreadPoolExecutor(max_workers=int(ingest_workers), thread_name_prefix="fvdb-ingest")
+                      if ingest_workers > 0 else None)
         self.queue: List[Request] = []      # arrived, waiting for memory
+        self.ingesting: List[tuple] = []    # (future, request), admission order
         self.prepared: List[Request] = []   # ingested, waiting for a batch
         self.inflight: List[tuple] = []     # (handle, requests)
         self.done: List[Request] = []
@@ -117,20 +128,44 @@ class ContinuousBatcher:
             self.reserved += need
             self.peak_reserved = max(self.peak_reserved, self.reserved)
             r.t_admit = self.now()
+            if self._pool is not None:
+                r.status = "ingesting"
+                self.ingesting.append((self._pool.submit(self.ex.prepare, r), r))
+                continue
             try:
                 self.ex.prepare(r)
             except Exception as e:  # noqa: BLE001 - a bad request fails alone
                 self._fail([r], e)
                 continue
-            # once ingested, the request's exact footprint is known: give back the slack of the
-            # admission bound (never more than was reserved)
-            exact = getattr(self.ex, "exact_bytes", None)
-            if exact is not None:
-                need = min(int(exact(r)), r.reserved)
-                self.reserved -= r.reserved - need
-                r.reserved = need
-            r.status = "prepared"
-            self.prepared.append(r)
+            self._ingested(r)
+
+    def _ingested(self, r: Request) -> None:
+        # once ingested, the request's exact footprint is known: give back the slack of the
+        # admission bound (never more than was reserved)
+        exact = getattr(self.ex, "exact_bytes", None)
+        if exact is not None:
+            need = min(int(exact(r)), r.reserved)
+            self.reserved -= r.reserved - need
+            r.reserved = need
+        r.status = "prepared"
+        r.t_prepared = self.now()
+        self.prepared.append(r)
+
+    def _collect_ingest(self) -> None:
+        # in admission order: a request waits for the ones admitted before it
+        while self.ingesting and self.ingesting[0][0].done():
+            fut, r = self.ingesting.pop(0)
+            err = fut.exception()
+            if err is not None:
+                self._fail([r], err)
+            else:
+                self._ingested(r)
+
+    def close(self) -> None:
+        """Stops the ingest threads (after the ingests in progress)."""
+        if self._pool is not None:
+            self._pool.shutdown(wait=True)
+            self._pool = None
 
     def _complete(self) -> None:
         still = []
@@ -167,8 +202,10 @@ class ContinuousBatcher:
                     break
                 batch.append(self.prepared.pop(0))
                 vox += n
+            t = self.now()
             for r in batch:
                 r.status = "running"
+                r.t_launch = t
             try:
                 handle = self.ex.launch(batch)
             except Exception as e:  # noqa: BLE001 - the batch fails, the server goes on
@@ -179,11 +216,12 @@ class ContinuousBatcher:
 
     def step(self) -> None:
         self._complete()
+        self._collect_ingest()
         self._admit()
         self._launch()
 
     def idle(self) -> bool:
-        return not (self.queue or self.prepared or self.inflight)
+        return not (self.queue or self.ingesting or self.prepared or self.inflight)
 
     def serve(self, requests: Sequence[Request], poll_sleep: float = 0.0) -> List[Request]:
         """Feed requests at their arrival times (seconds from now) until all are done."""
@@ -196,7 +234,12 @@ class ContinuousBatcher:
                 self.submit(pending[i])
                 i += 1
             self.step()
-            if poll_sleep:
+            if self.ingesting:
+                # a real sleep: a thread that only yields (sleep(0)) takes the GIL straight back,
+                # and the ingest thread then waits out the interpreter's switch interval (5 ms)
+                # after each device call it makes
+                time.sleep(max(poll_sleep, 2e-5))
+            elif poll_sleep:
                 time.sleep(poll_sleep)
         return sorted(self.done, key=lambda r: r.rid)
 
@@ -259,7 +302,7 @@ class UNetExecutor:
     """Runs a SparseUNet over batches of requests with device-resident per-request maps."""
 
     def __init__(self, model, dtype=torch.bfloat16, device="cuda", in_channels: Optional[int] = None,
-                 host_results: bool = False):
+                 host_results: bool = False, compute_streams: int = 3):
         import fvdb
 
         from ._autograd import weight_to_kio
@@ -272,7 +315,12 @@ class UNetExecutor:
         self.levels = len(model.channels)
         self.cin = in_channels or model.enc[0][0].in_channels
         self.ingest_stream = torch.cuda.Stream(device=self.device)
-        self.compute_stream = torch.cuda.Stream(device=self.device)
+        # a batch runs on a compute stream whose previous batch has finished, so with as many
+        # streams as batches in flight a small batch does not queue behind a large one on the
+        # device (every stream at the default priority)
+        self.compute_streams = [torch.cuda.Stream(device=self.device) for _ in range(max(1, int(compute_streams)))]
+        self._stream_done: List[Optional[torch.cuda.Event]] = [None] * len(self.compute_streams)
+        self._next_stream = 0
         # results go to pinned host memory on a stream of their own, so a batch's copy overlaps
         # the next batch's compute (every stream at the default priority)
         self.copy_stream = torch.cuda.Stream(device=self.device) if self.host_results else None
@@ -360,11 +408,25 @@ class UNetExecutor:
             ev.record(self.ingest_stream)
         req.state = {"topo": topo, "x": x, "ready": ev, "n0": grid.total_voxels}
 
+    def _pick_stream(self) -> int:
+        n = len(self.compute_streams)
+        for k in range(n):
+            i = (self._next_stream + k) % n
+            ev = self._stream_done[i]
+            if ev is None or ev.query():
+                break
+        else:
+            i = self._next_stream
+        self._next_stream = (i + 1) % n
+        return i
+
     def launch(self, reqs: Sequence[Request]):
         ext = self.fvdb.ext
-        with torch.cuda.stream(self.compute_stream):
+        si = self._pick_stream()
+        stream = self.compute_streams[si]
+        with torch.cuda.stream(stream):
             for r in reqs:
-                self.compute_stream.wait_event(r.state["ready"])
+                stream.wait_event(r.state["ready"])
             topos = [r.state["topo"] for r in reqs]
             L = self.levels
             offs = [[0] for _ in range(L)]
@@ -408,12 +470,13 @@ class UNetExecutor:
                 if b is not None:
                     x = x + b
             done = torch.cuda.Event()
-            done.record(self.compute_stream)
+            done.record(stream)
+            self._stream_done[si] = done
             # tensors made on the ingest stream are read here: tie their memory to this stream too
             for r in reqs:
                 tp = r.state["topo"]
                 for t in [r.state["x"]] + tp.conv + tp.down + tp.up:
-                    t.record_stream(self.compute_stream)
+                    t.record_stream(stream)
         if self.host_results:
             with torch.cuda.stream(self.copy_stream):
                 self.copy_stream.wait_event(done)

@@ -101,6 +101,25 @@ def test_executor_failure_fails_only_its_batch():
     assert b.reserved == 0
 
 
+@pytest.mark.parametrize("workers", [1, 2])
+def test_threaded_ingest_keeps_order_budget_and_failures(workers):
+    ex = FakeExecutor(bytes_per_voxel=1, done_after=2)
+    b = ContinuousBatcher(ex, hbm_budget_bytes=100, max_batch_voxels=50, max_inflight=2, ingest_workers=workers)
+    rs = reqs([40, 40, 40, 30, 20])
+    rs[2].seed = -1  # its ingest raises on the worker thread
+    out = b.serve(rs)
+    b.close()
+    assert [r.status for r in out] == ["done", "done", "failed", "done", "done"]
+    assert isinstance(out[2].error, RuntimeError) and b.reserved == 0 and b.peak_reserved <= 100
+    for r in out:
+        if r.status == "done":
+            assert torch.equal(r.result, torch.full((r.ijk.shape[0], 1), 2.0 * r.rid))
+            assert r.t_admit <= r.t_prepared <= r.t_launch <= r.t_done
+    # ingests complete in admission order: no request was prepared before an earlier admitted one
+    done = [r for r in out if r.status == "done"]
+    assert [r.rid for r in sorted(done, key=lambda r: r.t_prepared)] == [r.rid for r in sorted(done, key=lambda r: r.t_admit)]
+
+
 def test_latency_stats():
     rs = reqs([1, 1, 1, 1])
     for i, r in enumerate(rs):
@@ -165,6 +184,14 @@ def test_gpu_engine_matches_module_alone_and_in_mix():
         for a, m in zip(alone, mixed):
             assert a.status == "done" and m.status == "done"
             assert torch.equal(a.result, m.result), f"request {a.rid} differs alone vs in the mix"
+        # the serving configuration: ingest on a host thread, batches on three compute streams at
+        # once (staggered arrivals put the requests in separate batches that overlap on the device)
+        b = ContinuousBatcher(ex, hbm_budget_bytes=64 << 30, max_inflight=3, ingest_workers=1)
+        conc = b.serve([Request(rid=r.rid, ijk=r.ijk, seed=r.seed, arrival=2e-4 * r.rid) for r in rs])
+        b.close()
+        print(f"[serve] {dtype} concurrent batches {b.batches}", flush=True)
+        for a, c in zip(alone, conc):
+            assert c.status == "done" and torch.equal(a.result, c.result), f"request {a.rid} differs concurrently"
         # the engine runs the same kernels and maps as the fvdb.nn module
         mdl = model.to(dtype)
         with torch.no_grad():
@@ -195,7 +222,7 @@ def test_gpu_admission_bounds_measured_memory():
     torch.cuda.synchronize()
     base = torch.cuda.memory_allocated()
     torch.cuda.reset_peak_memory_stats()
-    b = ContinuousBatcher(ex, hbm_budget_bytes=budget, max_batch_voxels=4_000_000, max_inflight=2)
+    b = ContinuousBatcher(ex, hbm_budget_bytes=budget, max_batch_voxels=4_000_000, max_inflight=3, ingest_workers=1)
     waited = []
     orig = b._admit
 
@@ -204,6 +231,7 @@ def test_gpu_admission_bounds_measured_memory():
         waited.append(len(b.queue))
     b._admit = admit
     out = b.serve([Request(rid=r.rid, ijk=r.ijk, seed=r.seed) for r in rs])
+    b.close()
     torch.cuda.synchronize()
     peak = torch.cuda.max_memory_allocated() - base
     print(f"[serve budget] budget {budget / 1e9:.2f} GB, peak reserved by admission {b.peak_reserved / 1e9:.2f} GB, "
