@@ -27,6 +27,7 @@ executor. ``UNetExecutor`` is the GPU executor for ``fvdb.nn.SparseUNet``.
 from __future__ import annotations
 
 import math
+import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
@@ -314,7 +315,9 @@ class UNetExecutor:
         self.model = model
         self.levels = len(model.channels)
         self.cin = in_channels or model.enc[0][0].in_channels
-        self.ingest_stream = torch.cuda.Stream(device=self.device)
+        # one ingest stream per ingesting host thread, so concurrent ingests overlap on the device
+        # and each grid build's read-back waits only for its own work
+        self._ingest = threading.local()
         # a batch runs on a compute stream whose previous batch has finished, so with as many
         # streams as batches in flight a small batch does not queue behind a large one on the
         # device (every stream at the default priority)
@@ -393,9 +396,16 @@ class UNetExecutor:
         topo = req.state["topo"]
         return self._footprint(topo.n, topo.n[0])
 
+    def _ingest_stream(self) -> torch.cuda.Stream:
+        st = getattr(self._ingest, "stream", None)
+        if st is None:
+            st = self._ingest.stream = torch.cuda.Stream(device=self.device)
+        return st
+
     def prepare(self, req: Request) -> None:
         fvdb = self.fvdb
-        with torch.cuda.stream(self.ingest_stream):
+        ingest_stream = self._ingest_stream()
+        with torch.cuda.stream(ingest_stream):
             ijk = req.ijk.to(self.device, torch.int32, non_blocking=True)
             grid = fvdb.GridBatch.from_ijk(ijk)
             topo = UNetTopology(grid, self.levels)
@@ -405,7 +415,7 @@ class UNetExecutor:
                 gen = torch.Generator(device=self.device).manual_seed(int(req.seed))
                 x = torch.randn((grid.total_voxels, self.cin), generator=gen, device=self.device).to(self.dtype)
             ev = torch.cuda.Event()
-            ev.record(self.ingest_stream)
+            ev.record(ingest_stream)
         req.state = {"topo": topo, "x": x, "ready": ev, "n0": grid.total_voxels}
 
     def _pick_stream(self) -> int:
