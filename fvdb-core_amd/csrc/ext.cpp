@@ -455,6 +455,50 @@ at::Tensor implicit_gemm(at::Tensor in, at::Tensor a, at::Tensor map, int64_t n_
     return out;
 }
 
+// implicit_gemm over the channel concatenation [in0 | in1] without forming it where the
+// library has a two-source kernel for the shape (bit-identical to the concatenated call);
+// otherwise the features are concatenated and the one-source path runs.
+at::Tensor implicit_gemm_cat2(at::Tensor in0, at::Tensor in1, at::Tensor a, at::Tensor map, int64_t n_rows,
+                              c10::optional<at::Tensor> order, c10::optional<at::Tensor> bias, bool relu) {
+    require_hip(in0, "features");
+    require_hip(in1, "features");
+    TORCH_CHECK_VALUE(in0.dim() == 2 && in1.dim() == 2 && in0.size(0) == in1.size(0) && in0.size(1) == in1.size(1),
+                      "the two feature arrays must have the same shape [N, C]");
+    TORCH_CHECK_TYPE(in0.scalar_type() == in1.scalar_type(), "the two feature arrays must have the same dtype");
+    TORCH_CHECK_VALUE(in0.device() == in1.device(), "the two feature arrays must be on one device");
+    const bool fits = a.dim() == 3 && a.scalar_type() == in0.scalar_type() && a.size(2) == 2 * in0.size(1) &&
+                      map.dim() == 2 && map.scalar_type() == at::kInt && map.size(1) == a.size(0) &&
+                      map.size(0) >= n_rows && (!bias.has_value() || !bias->defined() ||
+                                                (bias->dim() == 1 && bias->size(0) == a.size(1) &&
+                                                 bias->device() == in0.device()));
+    if (fits) {
+        in0 = in0.contiguous();
+        in1 = in1.contiguous();
+        a = a.contiguous();
+        map = map.contiguous();
+        c10::hip::HIPGuard guard(in0.device().index());
+        const int32_t* ord = nullptr;
+        at::Tensor o;
+        if (order.has_value() && order->defined()) {
+            o = order->contiguous();
+            TORCH_CHECK_VALUE(o.scalar_type() == at::kInt && o.numel() == n_rows, "row order must be int32 [rows]");
+            ord = o.data_ptr<int32_t>();
+        }
+        at::Tensor b;
+        if (bias.has_value() && bias->defined()) b = bias->to(in0.scalar_type()).contiguous();
+        at::Tensor out = at::empty({n_rows, a.size(1)}, in0.options());
+        const int rc = fvdbk_conv_implicit_gemm_cat2(
+            to_dtype(in0.scalar_type()), in0.data_ptr(), in1.data_ptr(), in0.size(0), (int32_t)in0.size(1),
+            a.data_ptr(), map.data_ptr<int32_t>(), ord, n_rows, (int32_t)a.size(0), (int32_t)a.size(1),
+            b.defined() ? b.data_ptr() : nullptr, relu ? FVDBK_CONV_RELU : 0u, out.data_ptr(), stream_of(in0));
+        if (rc != FVDBK_EUNSUPPORTED) {
+            check_rc(rc, "conv_implicit_gemm_cat2");
+            return out;
+        }
+    }
+    return implicit_gemm(at::cat({in0, in1}, 1), a, map, n_rows, order, bias, relu);
+}
+
 // gw [K, Cin, Cout] (fp32, fp64 for fp64 inputs) from the neighbour-map pairs.
 // nbsizes_cpu: int64 CPU tensor [K] (pair counts per offset, known on the host).
 at::Tensor wgrad(at::Tensor in, at::Tensor gout, at::Tensor pairs, at::Tensor nbsizes_cpu, bool transposed,
@@ -611,6 +655,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("implicit_gemm", &implicit_gemm, nogil, pybind11::arg("in"), pybind11::arg("a"), pybind11::arg("map"),
           pybind11::arg("n_rows"), pybind11::arg("order") = pybind11::none(), pybind11::arg("bias") = pybind11::none(),
           pybind11::arg("relu") = false);
+    m.def("implicit_gemm_cat2", &implicit_gemm_cat2, nogil, py::arg("in0"), py::arg("in1"), py::arg("a"),
+          py::arg("map"), py::arg("n_rows"), py::arg("order") = py::none(), py::arg("bias") = py::none(),
+          py::arg("relu") = false);
     m.def("row_order", &row_order, nogil, py::arg("map"), py::arg("n_rows"), py::arg("window") = 0);
     m.def("wgrad", &wgrad, nogil);
     m.def("wgrad_tabled", &wgrad_tabled, nogil);

@@ -568,6 +568,41 @@ int fvdbk_conv_implicit_gemm_fused(fvdbk_dtype dtype, const void* in, int64_t n_
                       (flags & FVDBK_CONV_RELU) ? 1 : 0, out, stream);
 }
 
+int fvdbk_conv_implicit_gemm_cat2(fvdbk_dtype dtype, const void* in0, const void* in1, int64_t n_in, int32_t c_half,
+                                  const void* a, const int32_t* map, const int32_t* row_order, int64_t n_rows,
+                                  int32_t K, int32_t M, const void* bias, uint32_t flags, void* out,
+                                  hipStream_t stream) {
+    FVDBK_REQUIRE((flags & ~(uint32_t)FVDBK_CONV_RELU) == 0, FVDBK_EINVAL, "unknown epilogue flags 0x%x", flags);
+    FVDBK_REQUIRE(c_half > 0 && M > 0 && K > 0, FVDBK_EINVAL, "channel counts and kernel volume must be positive");
+    FVDBK_REQUIRE(n_in >= 0 && n_rows >= 0, FVDBK_EINVAL, "negative row count");
+    FVDBK_REQUIRE(n_rows < ((int64_t)1 << 31), FVDBK_EINVAL, "too many output rows");
+    if (n_rows == 0) return FVDBK_OK;
+    FVDBK_REQUIRE(in0 && in1 && a && map && out, FVDBK_EINVAL, "null pointer");
+    const int relu = (flags & FVDBK_CONV_RELU) ? 1 : 0;
+    bool launched = false;
+    switch (dtype) {
+    case FVDBK_BF16:
+        launched = conv_tile_dispatch2((const __bf16*)in0, (const __bf16*)in1, n_in, 2 * c_half, (const __bf16*)a, map,
+                                       row_order, n_rows, K, M, (__bf16*)out, (const __bf16*)bias, relu, stream);
+        break;
+    case FVDBK_F16:
+        launched = conv_tile_dispatch2((const _Float16*)in0, (const _Float16*)in1, n_in, 2 * c_half,
+                                       (const _Float16*)a, map, row_order, n_rows, K, M, (_Float16*)out,
+                                       (const _Float16*)bias, relu, stream);
+        break;
+    case FVDBK_F32:
+    case FVDBK_F64:
+        break;
+    default:
+        return set_error(FVDBK_ETYPE, "unsupported dtype %d", (int)dtype);
+    }
+    if (!launched)
+        return set_error(FVDBK_EUNSUPPORTED, "no two-source kernel for dtype %d, %d + %d -> %d channels, K = %d%s",
+                         (int)dtype, c_half, c_half, M, K, row_order ? " (row order)" : "");
+    FVDBK_LAUNCH_CHECK("k_conv_tile (two sources)");
+    return FVDBK_OK;
+}
+
 int fvdbk_kmap_row_keys(const int32_t* map, int64_t n_rows, int32_t K, int32_t* keys, hipStream_t stream) {
     FVDBK_REQUIRE(K > 0, FVDBK_EINVAL, "kernel volume must be positive");
     if (n_rows == 0) return FVDBK_OK;

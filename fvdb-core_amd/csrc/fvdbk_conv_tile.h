@@ -669,9 +669,11 @@ void k_conv_tile_dma(const T* __restrict__ X, int64_t n_in, const T* __restrict_
 // sequence, and guarded in the last groups, where the per-step load count, and with it the
 // wait before each barrier, depends on how many steps remain.  WPE waves per SIMD: 4
 // (128 registers per lane) or 2 (256, room for deeper gather lookahead).
-template <typename T, int LX, int NW, int KT, int MI, int NCC, int WPE = 4>
+// TWO (NCC = 2): the input is the channel concatenation [X | X1] of two arrays of 64
+// channels each, read from both without forming it (chunk 0 from X, chunk 1 from X1).
+template <typename T, int LX, int NW, int KT, int MI, int NCC, int WPE = 4, bool TWO = false>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
-void k_conv_tile_dma_perm(const T* __restrict__ X, int64_t n_in, const T* __restrict__ A,
+void k_conv_tile_dma_perm(const T* __restrict__ X, const T* __restrict__ X1, int64_t n_in, const T* __restrict__ A,
                           const int32_t* __restrict__ map, const int32_t* __restrict__ perm, int64_t n_rows, int M,
                           int n_mt, int64_t n_tiles, T* __restrict__ Y, const T* __restrict__ bias, int relu) {
     static_assert(sizeof(T) == 2, "16-bit features");
@@ -760,8 +762,12 @@ void k_conv_tile_dma_perm(const T* __restrict__ X, int64_t n_in, const T* __rest
 #pragma unroll
         for (int q = 0; q < SQ; ++q) lds_dma16(src + woff[q], dst + q * 512);
     };
-    const int bytes = (int)(n_in * CIN * (int64_t)sizeof(T));
+    static_assert(!TWO || NCC == 2, "two sources of one chunk each");
+    constexpr int RB = (TWO ? CC : CIN) * (int)sizeof(T);  // bytes per row of a source
+    const int bytes = (int)(n_in * RB);
     const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t xr1 =
+        __builtin_amdgcn_make_buffer_rsrc((void*)(TWO ? X1 : X), (short)0, bytes, 0x00020000);
     constexpr int MISSING = 0x7FFFFF00;  // beyond the features (checked by the launcher): loads zeros
     V b[LX + 1][NI][NCH];
     bool has[LX + 1][NI];
@@ -771,11 +777,12 @@ void k_conv_tile_dma_perm(const T* __restrict__ X, int64_t n_in, const T* __rest
 #pragma unroll
         for (int j = 0; j < NI; ++j) {
             const int32_t src = sMap[(wave * NI * 16 + j * 16 + r16) * KT + k];
-            const int off = src >= 0 ? src * (CIN * (int)sizeof(T)) + c * (CC * (int)sizeof(T)) + g * 16 : MISSING;
+            const int off = src >= 0 ? src * RB + (TWO ? 0 : c * (CC * (int)sizeof(T))) + g * 16 : MISSING;
             hh[j] = __any(src >= 0);
 #pragma unroll
             for (int ch = 0; ch < NCH; ++ch)
-                bb[j][ch] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(xr, off + ch * 64, 0, 0));
+                bb[j][ch] = __builtin_bit_cast(
+                    V, __builtin_amdgcn_raw_buffer_load_b128(TWO && c != 0 ? xr1 : xr, off + ch * 64, 0, 0));
         }
     };
     // one step; u = v mod U fixes the ring slots.  GUARD: the step may be among the last
@@ -971,10 +978,13 @@ static bool launch_tile_dma_wide(const T* X, int64_t n_in, const T* A, const int
 //    as in the other tile kernels, so the results are bit-identical to theirs.
 //  * The output tile goes through a wavefront-private LDS slice and leaves as whole rows.
 // Features must span fewer than 2^31 bytes (32-bit buffer offsets); callers check.
-template <typename T, int KT, int NCC, int LX>
+// TWO (NCC = 2): the input is the channel concatenation [X | X1] of two arrays of CC
+// channels each, read from both without forming it (chunk 0 from X, chunk 1 from X1).
+template <typename T, int KT, int NCC, int LX, bool TWO = false>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4)))
-void k_conv_resident(const T* __restrict__ X, int64_t n_in, const T* __restrict__ A, const int32_t* __restrict__ map,
-                     int64_t n_rows, int64_t n_wt, T* __restrict__ Y, const T* __restrict__ bias, int relu) {
+void k_conv_resident(const T* __restrict__ X, const T* __restrict__ X1, int64_t n_in, const T* __restrict__ A,
+                     const int32_t* __restrict__ map, int64_t n_rows, int64_t n_wt, T* __restrict__ Y,
+                     const T* __restrict__ bias, int relu) {
     static_assert(sizeof(T) == 2, "16-bit features");
     using F = Frag<T>;
     using V = typename F::vec;
@@ -1006,8 +1016,12 @@ void k_conv_resident(const T* __restrict__ X, int64_t n_in, const T* __restrict_
     const int64_t per = (n_wt + gridDim.x - 1) / gridDim.x;
     const int64_t t_begin = (int64_t)blockIdx.x * per;
     const int64_t t_end = t_begin + per < n_wt ? t_begin + per : n_wt;
-    const int bytes = (int)(n_in * CIN * (int64_t)sizeof(T));
+    static_assert(!TWO || NCC == 2, "two sources of one chunk each");
+    constexpr int RB = (TWO ? CC : CIN) * (int)sizeof(T);  // bytes per row of a source
+    const int bytes = (int)(n_in * RB);
     const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t xr1 =
+        __builtin_amdgcn_make_buffer_rsrc((void*)(TWO ? X1 : X), (short)0, bytes, 0x00020000);
     T* const slice = reinterpret_cast<T*>(res_smem + W_BYTES) + wave * SLICE;
 
     // map entries of tile t: lane (g, r16) of row group j holds row t*32 + j*16 + r16, taps
@@ -1046,13 +1060,15 @@ void k_conv_resident(const T* __restrict__ X, int64_t n_in, const T* __restrict_
 #pragma unroll
             for (int j = 0; j < NI; ++j) {
                 const int32_t src = __shfl(cur[j][k % TPL], (k / TPL) * 16 + r16);
-                const int off = src * (CIN * (int)sizeof(T)) + g * 16;
+                const int off = src * RB + g * 16;
 #pragma unroll
                 for (int c = 0; c < NCC; ++c) bb[j][c] = zero_frag<T>();
                 if (src >= 0) {
 #pragma unroll
                     for (int c = 0; c < NCC; ++c)
-                        bb[j][c] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(xr, off + c * 64, 0, 0));
+                        bb[j][c] = __builtin_bit_cast(
+                            V, TWO ? __builtin_amdgcn_raw_buffer_load_b128(c == 0 ? xr : xr1, off, 0, 0)
+                                   : __builtin_amdgcn_raw_buffer_load_b128(xr, off + c * 64, 0, 0));
                 }
             }
         };
@@ -1100,36 +1116,43 @@ static inline int device_cu_count() {
 // 0.45 -> 0.28 ms, 64 -> 32 0.92 -> 0.68 ms, 2x2x2 transposed 64 -> 32 0.29 -> 0.14 ms.
 template <typename T>
 static bool launch_resident(const T* X, int64_t n_in, int cin, const T* A, const int32_t* map, int64_t n_rows, int K,
-                            int M, T* Y, const T* bias, int relu, hipStream_t s) {
+                            int M, T* Y, const T* bias, int relu, hipStream_t s, const T* X1 = nullptr) {
     if (M != 32 || (cin != 32 && cin != 64) || (K != 27 && K != 8) ||
         n_in * cin * (int64_t)sizeof(T) > 0x7FFFFF00 || ((uintptr_t)Y % 16) != 0 || n_rows <= 0)
         return false;
+    if (X1 != nullptr && (cin != 64 || K != 27)) return false;  // two sources: the 3x3x3 64-channel form
     const int64_t n_wt = (n_rows + 31) / 32;
     const int64_t want = (n_wt + 15) / 16;
     const int n_cu = device_cu_count();
     const unsigned grid = (unsigned)(want < n_cu ? want : n_cu);
-    auto go = [&](auto kt, auto ncc, auto lx) {
+    auto go = [&](auto kt, auto ncc, auto lx, auto two) {
         constexpr int KT = decltype(kt)::value, NCC = decltype(ncc)::value, LX = decltype(lx)::value;
+        constexpr bool TWO = decltype(two)::value;
         const size_t lds = (size_t)KT * NCC * 128 * 16 + (size_t)16 * 2 * 16 * (32 + 8) * sizeof(T);
         // more than 64 KB of dynamic LDS: raise the kernel's limit once
         static const hipError_t attr = hipFuncSetAttribute(
-            reinterpret_cast<const void*>(&k_conv_resident<T, KT, NCC, LX>),
+            reinterpret_cast<const void*>(&k_conv_resident<T, KT, NCC, LX, TWO>),
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         (void)attr;
-        hipLaunchKernelGGL((k_conv_resident<T, KT, NCC, LX>), dim3(grid), dim3(1024), lds, s, X, n_in, A, map, n_rows,
-                           n_wt, Y, bias, relu);
+        hipLaunchKernelGGL((k_conv_resident<T, KT, NCC, LX, TWO>), dim3(grid), dim3(1024), lds, s, X, X1, n_in, A,
+                           map, n_rows, n_wt, Y, bias, relu);
     };
+    using F0 = std::false_type;
     using I1 = std::integral_constant<int, 1>;
     using I2 = std::integral_constant<int, 2>;
     using I4 = std::integral_constant<int, 4>;
     using I8 = std::integral_constant<int, 8>;
     using I27 = std::integral_constant<int, 27>;
+    if (X1 != nullptr) {
+        go(I27{}, I2{}, I1{}, std::true_type{});
+        return true;
+    }
     if (K == 27) {
-        if (cin == 32) go(I27{}, I1{}, std::integral_constant<int, 3>{});
-        else go(I27{}, I2{}, I1{});
+        if (cin == 32) go(I27{}, I1{}, std::integral_constant<int, 3>{}, F0{});
+        else go(I27{}, I2{}, I1{}, F0{});
     } else {
-        if (cin == 32) go(I8{}, I1{}, I4{});
-        else go(I8{}, I2{}, I2{});
+        if (cin == 32) go(I8{}, I1{}, I4{}, F0{});
+        else go(I8{}, I2{}, I2{}, F0{});
     }
     return true;
 }
@@ -1142,33 +1165,41 @@ static bool launch_resident(const T* X, int64_t n_in, int cin, const T* A, const
 template <typename T>
 static bool launch_tile_dma_perm(const T* X, int64_t n_in, int cin, const T* A, const int32_t* map,
                                  const int32_t* perm, int64_t n_rows, int K, int M, T* Y, const T* bias, int relu,
-                                 hipStream_t s) {
+                                 hipStream_t s, const T* X1 = nullptr) {
     if (K != 27 || (cin != 64 && cin != 128) || M % 64 != 0 || n_in * cin * (int64_t)sizeof(T) > 0x7FFFFF00 ||
         ((uintptr_t)Y % 16) != 0)
         return false;
+    if (X1 != nullptr && cin != 128) return false;  // two sources: the 128-channel (2 x 64) form
     constexpr int NW = 8, KT = 27;
     const size_t map_bytes = (size_t)32 * NW * KT * 4;
-    auto go = [&](auto li, auto mi, auto ncc, auto wpe) {
+    auto go = [&](auto li, auto mi, auto ncc, auto wpe, auto two) {
         constexpr int LX = decltype(li)::value, MI = decltype(mi)::value, NCC = decltype(ncc)::value;
         constexpr int WPE = decltype(wpe)::value;
+        constexpr bool TWO = decltype(two)::value;
         const int n_mt = M / (MI * 16);
         const int64_t n_tiles = (n_rows + 32 * NW - 1) / (32 * NW) * n_mt;
         const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
         const size_t out_bytes = MI == 4 ? (size_t)(NW - 3) * 2 * 16 * (64 + 8) * sizeof(T) : 0;
-        hipLaunchKernelGGL((k_conv_tile_dma_perm<T, LX, NW, KT, MI, NCC, WPE>), grid, dim3(64 * NW),
-                           map_bytes > out_bytes ? map_bytes : out_bytes, s, X, n_in, A, map, perm, n_rows, M, n_mt,
-                           n_tiles, Y, bias, relu);
+        hipLaunchKernelGGL((k_conv_tile_dma_perm<T, LX, NW, KT, MI, NCC, WPE, TWO>), grid, dim3(64 * NW),
+                           map_bytes > out_bytes ? map_bytes : out_bytes, s, X, X1, n_in, A, map, perm, n_rows, M,
+                           n_mt, n_tiles, Y, bias, relu);
     };
     using I1 = std::integral_constant<int, 1>;
     using I2 = std::integral_constant<int, 2>;
     using I4 = std::integral_constant<int, 4>;
     using I8 = std::integral_constant<int, 8>;
+    using F0 = std::false_type;
     if (cin == 128) {
-        if (M % 128 == 0) go(I1{}, I8{}, I2{}, I4{});
-        else go(I2{}, I4{}, I2{}, I4{});
+        if (X1 != nullptr) {
+            if (M % 128 == 0) go(I1{}, I8{}, I2{}, I4{}, std::true_type{});
+            else go(I2{}, I4{}, I2{}, I4{}, std::true_type{});
+        } else {
+            if (M % 128 == 0) go(I1{}, I8{}, I2{}, I4{}, F0{});
+            else go(I2{}, I4{}, I2{}, I4{}, F0{});
+        }
     } else {
-        if (M % 128 == 0) go(I1{}, I8{}, I1{}, I4{});
-        else go(I2{}, I4{}, I1{}, I4{});
+        if (M % 128 == 0) go(I1{}, I8{}, I1{}, I4{}, F0{});
+        else go(I2{}, I4{}, I1{}, I4{}, F0{});
     }
     return true;
 }
@@ -1186,11 +1217,13 @@ static bool launch_tile_dma_perm(const T* X, int64_t n_in, int cin, const T* A, 
 // costs neither its gathers, its weight fetch, its barrier nor its MFMAs.  A skipped tap
 // would have added exact zeros, so the result is bit-identical to the full sequence
 // for finite weights.
-template <typename T, int MI, int NI, int CC, int NCC, int KT, int LX, int WPE, bool SKIP = false>
+// TWO: the input is the channel concatenation [X | X1] of two arrays of CIN / 2 channels,
+// read from both without forming it (chunks 0 .. NCC/2-1 from X, the rest from X1).
+template <typename T, int MI, int NI, int CC, int NCC, int KT, int LX, int WPE, bool SKIP = false, bool TWO = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, WPE > 0 ? WPE : 8)))
 void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32_t* __restrict__ map,
                    const int32_t* __restrict__ perm, int64_t n_rows, int M, int n_mt, int64_t n_tiles,
-                   T* __restrict__ Y, const T* __restrict__ bias, int relu) {
+                   T* __restrict__ Y, const T* __restrict__ bias, int relu, const T* __restrict__ X1 = nullptr) {
     using F = Frag<T>;
     using V = typename F::vec;
     constexpr int CIN = CC * NCC;
@@ -1294,15 +1327,19 @@ void k_conv_tile_c(const T* __restrict__ X, const T* __restrict__ A, const int32
         }
     };
     // gathered input rows of step v (called once per step, in step order)
+    static_assert(!TWO || NCC % 2 == 0, "two sources of NCC / 2 chunks each");
+    constexpr int XS = TWO ? CIN / 2 : CIN;  // row stride of a source
     auto gather = [&](int v, V (&bb)[NI][NCH]) {
-        const int tap = next_tap(git, v), c0 = (v & (NCC - 1)) * CC;
+        const int tap = next_tap(git, v), cc = v & (NCC - 1);
+        const T* const xs = TWO && cc >= NCC / 2 ? X1 : X;
+        const int c0 = (TWO ? cc & (NCC / 2 - 1) : cc) * CC;
 #pragma unroll
         for (int j = 0; j < NI; ++j) {
             const int32_t src = sMap[(wave * NI * 16 + j * 16 + r16) * KT + tap];
 #pragma unroll
             for (int ch = 0; ch < NCH; ++ch) bb[j][ch] = zero_frag<T>();
             if (src >= 0) {
-                const T* xr = X + (int64_t)src * CIN + c0 + g * F::EPL;
+                const T* xr = xs + (int64_t)src * XS + c0 + g * F::EPL;
 #pragma unroll
                 for (int ch = 0; ch < NCH; ++ch) bb[j][ch] = *reinterpret_cast<const V*>(xr + ch * F::CH);
             }
@@ -1484,6 +1521,29 @@ static void launch_tile_c(const T* X, const T* A, const int32_t* map, const int3
                            map, perm, n_rows, M, n_mt, n_tiles, Y, bias, relu);
 }
 
+// Two-source form of launch_tile_c for 16-bit 3x3x3 layers of 256 input channels (two
+// sources of 128) and more than 32 (not 128k) output channels: the 64-channel-tile
+// instance the single-source dispatch picks for that shape, reading [X | X1].
+template <typename T>
+static bool try_tile_c_two(const T* X, const T* X1, const T* A, const int32_t* map, const int32_t* perm,
+                           int64_t n_rows, int K, int M, T* Y, const T* bias, int relu, hipStream_t s) {
+    constexpr int MI = 4, NI = 2, NCC = 4, CC = 64;
+    if (sizeof(T) != 2 || K != 27 || M <= 32) return false;
+    const int64_t n_rt = (n_rows + 64 * NI - 1) / (64 * NI);
+    const int n_mt = (M + MI * 16 - 1) / (MI * 16);
+    const int64_t n_tiles = n_rt * n_mt;
+    const dim3 grid((unsigned)((n_tiles + 7) / 8 * 8));
+    const size_t lds = tile_lds_bytes<T, MI, NI, CC>(K);
+    constexpr int LX27 = 1, WPE = 4;  // the single-source choice for this shape (W4_27)
+    if (perm)
+        hipLaunchKernelGGL((k_conv_tile_c<T, MI, NI, CC, NCC, 27, LX27, WPE, true, true>), grid, dim3(256), lds, s, X,
+                           A, map, perm, n_rows, M, n_mt, n_tiles, Y, bias, relu, X1);
+    else
+        hipLaunchKernelGGL((k_conv_tile_c<T, MI, NI, CC, NCC, 27, LX27, WPE, false, true>), grid, dim3(256), lds, s,
+                           X, A, map, perm, n_rows, M, n_mt, n_tiles, Y, bias, relu, X1);
+    return true;
+}
+
 // 16-bit outputs in multiples of 128 channels take 128-channel tiles (each gathered
 // row feeds twice the MFMAs) where that measured faster: two chunks per tap, or 8 taps.
 template <typename T>
@@ -1556,6 +1616,30 @@ inline bool conv_tile_dispatch_t(const T* X, int64_t n_in, int cin, const T* A, 
     }
 }
 
+// Contraction over the channel concatenation [X | X1] of two arrays of cin / 2 channels
+// each, without forming it, where the kernel the single-source dispatch picks for the
+// shape has a two-source form (16-bit 3x3x3 layers: 64 inputs and 32 outputs in tree
+// order, 128 inputs through a row order, 256 inputs and more than 32 outputs); false
+// otherwise.  Same instance and accumulation order as for the concatenated input, so the
+// result is bit-identical to it.
+template <typename T>
+inline bool conv_tile_dispatch2_t(const T* X, const T* X1, int64_t n_in, int cin, const T* A, const int32_t* map,
+                                  const int32_t* perm, int64_t n_rows, int K, int M, T* Y, const T* bias, int relu,
+                                  hipStream_t s) {
+    if constexpr (sizeof(T) != 2) {
+        return false;
+    } else {
+        if (K != 27 || ((uintptr_t)X % 16) != 0 || ((uintptr_t)X1 % 16) != 0 || ((uintptr_t)A % 16) != 0)
+            return false;
+        if (perm == nullptr && cin == 64 && M == 32)
+            return launch_resident<T>(X, n_in, cin, A, map, n_rows, K, M, Y, bias, relu, s, X1);
+        if (perm != nullptr && cin == 128)
+            return launch_tile_dma_perm<T>(X, n_in, cin, A, map, perm, n_rows, K, M, Y, bias, relu, s, X1);
+        if (cin == 256) return try_tile_c_two<T>(X, X1, A, map, perm, n_rows, K, M, Y, bias, relu, s);
+        return false;
+    }
+}
+
 // Per element type (one translation unit each): launches the LDS-tiled kernel and
 // returns true, or returns false when the shape has no tiled instance.
 bool conv_tile_dispatch(const __bf16* X, int64_t n_in, int cin, const __bf16* A, const int32_t* map, const int32_t* perm,
@@ -1564,5 +1648,11 @@ bool conv_tile_dispatch(const _Float16* X, int64_t n_in, int cin, const _Float16
                         int64_t n_rows, int K, int M, _Float16* Y, const _Float16* bias, int relu, hipStream_t s);
 bool conv_tile_dispatch(const float* X, int64_t n_in, int cin, const float* A, const int32_t* map, const int32_t* perm,
                         int64_t n_rows, int K, int M, float* Y, const float* bias, int relu, hipStream_t s);
+bool conv_tile_dispatch2(const __bf16* X, const __bf16* X1, int64_t n_in, int cin, const __bf16* A,
+                         const int32_t* map, const int32_t* perm, int64_t n_rows, int K, int M, __bf16* Y,
+                         const __bf16* bias, int relu, hipStream_t s);
+bool conv_tile_dispatch2(const _Float16* X, const _Float16* X1, int64_t n_in, int cin, const _Float16* A,
+                         const int32_t* map, const int32_t* perm, int64_t n_rows, int K, int M, _Float16* Y,
+                         const _Float16* bias, int relu, hipStream_t s);
 
 }  // namespace fvdbk

@@ -8,4 +8,9 @@ bool conv_tile_dispatch(const _Float16* X, int64_t n_in, int cin, const _Float16
     return conv_tile_dispatch_t<_Float16>(X, n_in, cin, A, map, perm, n_rows, K, M, Y, bias, relu, s);
 }
 
+bool conv_tile_dispatch2(const _Float16* X, const _Float16* X1, int64_t n_in, int cin, const _Float16* A,
+                         const int32_t* map, const int32_t* perm, int64_t n_rows, int K, int M, _Float16* Y,
+                         const _Float16* bias, int relu, hipStream_t s) {
+    return conv_tile_dispatch2_t<_Float16>(X, X1, n_in, cin, A, map, perm, n_rows, K, M, Y, bias, relu, s);
+}
 }  // namespace fvdbk

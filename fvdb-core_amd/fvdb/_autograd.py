@@ -20,6 +20,8 @@ mapped back with the inverse permutations (``:170-187``).
 """
 from __future__ import annotations
 
+from typing import Optional
+
 import torch
 
 from ._lib import ext, require_device
@@ -74,8 +76,14 @@ def wgrad_chunk(n_pairs: int, dtype: torch.dtype = torch.float32, cin: int = 128
     return int(max(256, c))
 
 
-def check_conv_args(feats: torch.Tensor, weight: torch.Tensor, n_in: int, transposed: bool) -> None:
-    """Input checks of ``autograd/SparseConvolutionKernelMap.cpp:34-93``."""
+def check_conv_args(feats: torch.Tensor, weight: torch.Tensor, n_in: int, transposed: bool,
+                    feats2: Optional[torch.Tensor] = None) -> None:
+    """Input checks of ``autograd/SparseConvolutionKernelMap.cpp:34-93``; with `feats2` the
+    input is the channel concatenation [feats | feats2] (same shape and dtype)."""
+    if feats2 is not None and (feats2.shape != feats.shape or feats2.dtype != feats.dtype or
+                               feats2.device != feats.device):
+        raise ValueError(f"concatenated features must match the features: {tuple(feats2.shape)} {feats2.dtype} "
+                         f"vs {tuple(feats.shape)} {feats.dtype}")
     if not feats.is_floating_point():
         raise TypeError("features must have a floating point type")
     if feats.dim() != 2:
@@ -90,9 +98,9 @@ def check_conv_args(feats: torch.Tensor, weight: torch.Tensor, n_in: int, transp
     if weight.dim() != 5:
         raise ValueError(f"Expected kernels to have 5 dimensions but got {weight.dim()} dimensions")
     cin = weight.shape[0] if transposed else weight.shape[1]
-    if cin != feats.shape[1]:
-        raise ValueError(f"Expected input channels of kernels ({cin}) to equal input channels of features: "
-                         f"{feats.shape[1]}")
+    nf = feats.shape[1] * (2 if feats2 is not None else 1)
+    if cin != nf:
+        raise ValueError(f"Expected input channels of kernels ({cin}) to equal input channels of features: {nf}")
     if weight.dtype != feats.dtype:
         raise TypeError(f"features ({feats.dtype}) and kernels ({weight.dtype}) must have the same dtype")
     require_device(feats, "features")
@@ -103,17 +111,21 @@ class SparseConvFunction(torch.autograd.Function):
     contraction's epilogue (one rounding to the feature dtype); bias may be None."""
 
     @staticmethod
-    def forward(ctx, feats, weight, bias, pack, transposed: bool, relu: bool = False):
+    def forward(ctx, feats, weight, bias, pack, transposed: bool, relu: bool = False, feats2=None):
         n_src, n_tgt = pack.source_grid.total_voxels, pack.target_grid.total_voxels
-        cin, dt = feats.shape[1], feats.dtype
+        cin, dt = feats.shape[1] * (2 if feats2 is not None else 1), feats.dtype
         koi = weight_to_koi(weight, transposed)
         if not transposed:
             fmap, order, n_out = pack._map_fwd(), pack._order_for(True, cin, dt, koi.shape[1]), n_tgt
         else:
             fmap, order, n_out = pack._map_bwd(), pack._order_for(False, cin, dt, koi.shape[1]), n_src
-        out = ext.implicit_gemm(feats.contiguous(), koi, fmap, n_out, order, bias, bool(relu))
+        if feats2 is None:
+            out = ext.implicit_gemm(feats.contiguous(), koi, fmap, n_out, order, bias, bool(relu))
+        else:
+            # input channels [feats | feats2], read from both without forming the concatenation
+            out = ext.implicit_gemm_cat2(feats, feats2, koi, fmap, n_out, order, bias, bool(relu))
         # the [K, I, O] operand of dgrad is built from the weight in backward, when needed
-        ctx.save_for_backward(feats, weight, out if relu else None)
+        ctx.save_for_backward(feats, weight, out if relu else None, feats2)
         ctx.pack = pack
         ctx.transposed = transposed
         ctx.relu = bool(relu)
@@ -122,7 +134,7 @@ class SparseConvFunction(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_out):
-        feats, weight, out = ctx.saved_tensors
+        feats, weight, out, feats2 = ctx.saved_tensors
         pack, transposed = ctx.pack, ctx.transposed
         kio = weight_to_kio(weight, transposed)  # [K, I, O]
         grad_out = grad_out.contiguous()
@@ -134,8 +146,8 @@ class SparseConvFunction(torch.autograd.Function):
         if ctx.needs_input_grad[2]:
             gb = grad_out.sum(0, dtype=torch.float64 if grad_out.dtype == torch.float64 else torch.float32)
             gb = gb.to(ctx.bias_dtype)
-        gin = gw = None
-        if ctx.needs_input_grad[0]:
+        gin = gw = gin2 = None
+        if ctx.needs_input_grad[0] or (feats2 is not None and ctx.needs_input_grad[6]):
             cg, dt = grad_out.shape[1], grad_out.dtype
             co = kio.shape[1]  # the dgrad contraction's output channels (the layer's inputs)
             if not transposed:
@@ -143,20 +155,26 @@ class SparseConvFunction(torch.autograd.Function):
             else:
                 dmap, order, n_in = pack._map_fwd(), pack._order_for(True, cg, dt, co), pack.target_grid.total_voxels
             gin = ext.implicit_gemm(grad_out, kio, dmap, n_in, order)
+            if feats2 is not None:  # the gradient of each half of the concatenation
+                c = feats.shape[1]
+                gin, gin2 = gin[:, :c], gin[:, c:]
         if ctx.needs_input_grad[1]:
+            x = feats if feats2 is None else torch.cat([feats, feats2], 1)
             pairs, _ = pack._pairs()
-            chunk = wgrad_chunk(pairs.shape[0], feats.dtype, feats.shape[1], grad_out.shape[1])
+            chunk = wgrad_chunk(pairs.shape[0], x.dtype, x.shape[1], grad_out.shape[1])
             koff, cstart, total = pack._wgrad_tables(chunk)
-            gk = ext.wgrad_tabled(feats.contiguous(), grad_out, pairs, koff, cstart, total, transposed, chunk)
+            gk = ext.wgrad_tabled(x.contiguous(), grad_out, pairs, koff, cstart, total, transposed, chunk)
             gw = kio_to_weight(gk.to(kio.dtype), pack.kernel_size, transposed)
-        return gin, gw, gb, None, None, None
+        return gin, gw, gb, None, None, None, gin2
 
 
 def sparse_conv(feats: torch.Tensor, weight: torch.Tensor, pack, transposed: bool, bias=None,
-                relu: bool = False) -> torch.Tensor:
+                relu: bool = False, feats2: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """The convolution of `feats`, or with `feats2` of the channel concatenation
+    [feats | feats2] without forming it."""
     n_in = pack.target_grid.total_voxels if transposed else pack.source_grid.total_voxels
-    check_conv_args(feats, weight, n_in, transposed)
+    check_conv_args(feats, weight, n_in, transposed, feats2)
     cout = weight.shape[1] if transposed else weight.shape[0]
     if bias is not None and (bias.dim() != 1 or bias.shape[0] != cout):
         raise ValueError(f"bias must have shape [{cout}], got {tuple(bias.shape)}")
-    return SparseConvFunction.apply(feats, weight, bias, pack, transposed, relu)
+    return SparseConvFunction.apply(feats, weight, bias, pack, transposed, relu, feats2)

@@ -21,7 +21,7 @@ import torch.nn as nn
 from torch.profiler import record_function
 
 from ..grid_batch import GridBatch
-from ..jagged_tensor import JaggedTensor
+from ..jagged_tensor import JaggedTensor, jcat_tensors
 from ..sparse_conv_pack_info import ConvPackBackend, SparseConvPackInfo
 from .vdbtensor import VDBTensor
 
@@ -195,9 +195,13 @@ class SparseConv3d(nn.Module):
             return ConvPackBackend.LGGS
         raise NotImplementedError(f"Backend {backend} is not supported")
 
-    def _dispatch_conv(self, in_feature: JaggedTensor, in_grid: GridBatch, in_kmap, out_grid, relu: bool = False):
+    def _dispatch_conv(self, in_feature: JaggedTensor, in_grid: GridBatch, in_kmap, out_grid, relu: bool = False,
+                       concat: Optional[JaggedTensor] = None):
         """(out_grid, out_feature, out_kmap, fused): fused tells whether bias and ReLU
-        were applied in the convolution's epilogue."""
+        were applied in the convolution's epilogue.  `concat`: features whose channels follow
+        in_feature's (the implicit-GEMM backends read both without concatenating them)."""
+        if concat is not None and (self.transposed or self.backend in ("halo", "dense")):
+            in_feature, concat = jcat_tensors([in_feature, concat], dim=1), None
         backend = self.backend
         if backend == "cutlass" and (self.in_channels, self.out_channels) not in self.CUTLASS_SUPPORTED_CHANNELS:
             backend = "default"
@@ -230,22 +234,35 @@ class SparseConv3d(nn.Module):
         out_kmap = kmap if can_cache else None
         be = self._build_kmap_and_convert_backend(kmap, backend)
         if not self.transposed:
-            out_feature = kmap.sparse_conv_3d(in_feature, self.weight, be, bias=self.bias, relu=relu)
+            out_feature = kmap.sparse_conv_3d(in_feature, self.weight, be, bias=self.bias, relu=relu, concat=concat)
         else:
             out_feature = kmap.sparse_transpose_conv_3d(in_feature, self.weight, be, bias=self.bias, relu=relu)
         return out_grid, out_feature, out_kmap, True
 
-    def forward(self, input: VDBTensor, out_grid: Optional[GridBatch] = None, *, fuse_relu: bool = False) -> VDBTensor:
+    def forward(self, input: VDBTensor, out_grid: Optional[GridBatch] = None, *, fuse_relu: bool = False,
+                concat: Optional[VDBTensor] = None) -> VDBTensor:
         """``fuse_relu`` applies a ReLU in the convolution's epilogue (same values as a
-        following ``fvdb.nn.ReLU``, without a separate pass over the output)."""
+        following ``fvdb.nn.ReLU``, without a separate pass over the output).  ``concat`` (a
+        VDBTensor on the input's grid): the layer convolves ``fvdb.jcat([input, concat],
+        dim=1)``, the U-Net decoder's skip connection, read from both tensors without
+        forming the concatenation (same result)."""
         in_feature, in_grid, in_kmap = input.data, input.grid, input.kmap
+        cat = None
+        if concat is not None:
+            if not (concat.grid is in_grid or concat.grid.is_same(in_grid)):
+                raise ValueError("concat must be on the input's grid")
+            cat = concat.data
+            if in_kmap is None:
+                in_kmap = concat.kmap
         fused = False
+        if cat is not None and self.kernel_size == (1, 1, 1) and self.stride == (1, 1, 1):
+            in_feature, cat = jcat_tensors([in_feature, cat], dim=1), None
         if self.kernel_size == (1, 1, 1) and self.stride == (1, 1, 1):
             out_feature = in_feature.jagged_like(in_feature.jdata.matmul(self.weight.transpose(0, 1)))
             out_grid, out_kmap = in_grid, in_kmap
         else:
             out_grid, out_feature, out_kmap, fused = self._dispatch_conv(in_feature, in_grid, in_kmap, out_grid,
-                                                                         fuse_relu)
+                                                                         fuse_relu, cat)
         if not fused:
             if self.bias is not None:
                 out_feature.jdata = out_feature.jdata + self.bias

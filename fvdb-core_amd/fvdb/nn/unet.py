@@ -6,7 +6,8 @@ Structure (levels l = 0..L-1, channels c_l, default 32/64/128/256):
 * encoder level l>0: SparseConv3d(c_{l-1} -> c_l, 2, stride 2) down, then two 3x3x3 convs
 * decoder level l<L-1: SparseConv3d(c_{l+1} -> c_l, 2, stride 2, transposed) onto the level-l
   grid, ``fvdb.jcat([up, skip_l], dim=1)`` (2 c_l channels), then two 3x3x3 convs
-  (2 c_l -> c_l -> c_l)
+  (2 c_l -> c_l -> c_l); the first reads the concatenation's two halves from ``up`` and
+  ``skip_l`` (``SparseConv3d(..., concat=skip_l)``) instead of forming it
 * optional 1x1x1 head
 
 A ReLU follows every convolution except the head; it and the bias run in the
@@ -49,8 +50,6 @@ class SparseUNet(nn.Module):
         self.relu = ReLU()
 
     def forward(self, x: VDBTensor) -> VDBTensor:
-        import fvdb
-
         skips: List[VDBTensor] = []
         for l, convs in enumerate(self.enc):
             if l > 0:
@@ -62,9 +61,10 @@ class SparseUNet(nn.Module):
         for i, l in enumerate(range(L - 2, -1, -1)):
             skip = skips[l]
             x = self.up[i](x, out_grid=skip.grid, fuse_relu=True)
-            x = fvdb.jcat([x, skip], dim=1)
-            x = VDBTensor(x.grid, x.data, skip.kmap)  # same grid: reuse its stride-1 kernel map
-            for conv in self.dec[i]:
+            x = VDBTensor(skip.grid, x.data, skip.kmap)  # same grid: reuse its stride-1 kernel map
+            # the first decoder convolution takes fvdb.jcat([x, skip], dim=1), read from both
+            x = self.dec[i][0](x, fuse_relu=True, concat=skip)
+            for conv in self.dec[i][1:]:
                 x = conv(x, fuse_relu=True)
         if self.head is not None:
             x = self.head(x)

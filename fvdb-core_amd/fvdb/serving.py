@@ -450,14 +450,17 @@ class UNetExecutor:
 
             orders = {}
 
-            def layer(inp, wb, m, n_rows):
+            def layer(inp, wb, m, n_rows, cat=None):
                 a, b = wb
                 # rows grouped by tap mask per window where that pays (same rule as the packs)
                 order = None
-                if n_rows > 0 and wants_row_order(m.shape[1], inp.shape[1], inp.dtype, a.shape[1], n_rows):
+                cin = inp.shape[1] * (2 if cat is not None else 1)
+                if n_rows > 0 and wants_row_order(m.shape[1], cin, inp.dtype, a.shape[1], n_rows):
                     if id(m) not in orders:
                         orders[id(m)] = ext.row_order(m, n_rows, AUTO_ORDER_WINDOW)
                     order = orders[id(m)]
+                if cat is not None:  # input channels [inp | cat], read from both
+                    return ext.implicit_gemm_cat2(inp, cat, a, m, n_rows, order, b, True)
                 return ext.implicit_gemm(inp, a, m, n_rows, order, b, True)  # bias + ReLU in the epilogue
 
             skips = []
@@ -471,8 +474,8 @@ class UNetExecutor:
             for i, l in enumerate(range(L - 2, -1, -1)):
                 n_l = offs[l][-1]
                 x = layer(x, self.up[i], up[l], n_l)
-                x = torch.cat([x, skips[l]], 1)
-                for wb in self.dec[i]:
+                x = layer(x, self.dec[i][0], conv[l], n_l, cat=skips[l])  # the skip concatenation
+                for wb in self.dec[i][1:]:
                     x = layer(x, wb, conv[l], n_l)
             if self.head is not None:
                 w, b = self.head

@@ -331,14 +331,20 @@ class SparseConvPackInfo:
 
     def sparse_conv_3d(self, input, weights: torch.Tensor,
                        backend: ConvPackBackend = ConvPackBackend.GATHER_SCATTER, *, bias=None,
-                       relu: bool = False) -> JaggedTensor:
+                       relu: bool = False, concat=None) -> JaggedTensor:
+        """``concat`` (same layout as ``input``): convolve the channel concatenation
+        [input | concat], as ``fvdb.jcat([input, concat], dim=1)`` would form it, without
+        forming it (16-bit layers read both arrays in the contraction; bit-identical)."""
         j = input if isinstance(input, JaggedTensor) else JaggedTensor(input)
         if j.num_tensors != self._src.grid_count:
             raise ValueError("Input batch size must match target grid batch size")
         if j.element_count != self._src.total_voxels:
             raise ValueError("Input element count must match target grid total voxels")
         self._check_backend(backend, False)
-        out = sparse_conv(j.jdata, weights, self, False, bias, relu)
+        f2 = None
+        if concat is not None:
+            f2 = concat.jdata if isinstance(concat, JaggedTensor) else concat
+        out = sparse_conv(j.jdata, weights, self, False, bias, relu, f2)
         return self._tgt.jagged_like(out)
 
     def sparse_transpose_conv_3d(self, input, weights: torch.Tensor,

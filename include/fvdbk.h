@@ -278,6 +278,21 @@ int fvdbk_conv_implicit_gemm_fused(fvdbk_dtype dtype, const void* in, int64_t n_
                                    const int32_t* map, const int32_t* row_order, int64_t n_rows, int32_t K, int32_t M,
                                    const void* bias, uint32_t flags, void* out, hipStream_t stream);
 
+/* The contraction of fvdbk_conv_implicit_gemm_fused over the channel concatenation
+ * [in0 | in1] of two [n_in][c_half] feature arrays (c_in = 2 c_half), read from both
+ * without forming it.  Replaces fvdb.jcat([a, b], dim=1) followed by the convolution of
+ * the concatenation (the decoder pattern of docs/tutorials/simple_unet.md:200-245), which
+ * writes and re-reads the concatenated features.  Where the kernel that
+ * fvdbk_conv_implicit_gemm_fused picks for the shape has a two-source form (16-bit 3x3x3
+ * layers: c_half = 32 with M = 32 and no row order, c_half = 64 with a row order,
+ * c_half = 128 with M > 32) the result is bit-identical to that call on the concatenated
+ * input; otherwise the call launches nothing and returns FVDBK_EUNSUPPORTED, and the
+ * caller concatenates. */
+int fvdbk_conv_implicit_gemm_cat2(fvdbk_dtype dtype, const void* in0, const void* in1, int64_t n_in, int32_t c_half,
+                                  const void* a, const int32_t* map, const int32_t* row_order, int64_t n_rows,
+                                  int32_t K, int32_t M, const void* bias, uint32_t flags, void* out,
+                                  hipStream_t stream);
+
 /* keys[r] = bit k set when map[r][k] >= 0, for the first min(K, 31) taps (the
  * reference's bitmask of IGEMMBitOperations.cu:14-36 with one split). */
 int fvdbk_kmap_row_keys(const int32_t* map, int64_t n_rows, int32_t K, int32_t* keys, hipStream_t stream);
