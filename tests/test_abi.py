@@ -61,6 +61,24 @@ def test_extension_error_classes_on_cpu_tensors():
         e.implicit_gemm(torch.zeros(3, 4), torch.zeros(1, 2, 4), torch.zeros(3, 1, dtype=torch.int32), 3)
     with pytest.raises(ValueError):
         fvdb.GridBatch.from_ijk(torch.zeros(4, 3, dtype=torch.int32))  # CPU tensors: no CPU path
+    with pytest.raises(ValueError):
+        e.implicit_gemm_cat2(torch.zeros(3, 2), torch.zeros(3, 2), torch.zeros(1, 2, 4),
+                             torch.zeros(3, 1, dtype=torch.int32), 3)
+
+
+def test_concat_argument_checks():
+    from fvdb._autograd import check_conv_args
+
+    w = torch.zeros(8, 6, 3, 3, 3)
+    # [3 | 3] = 6 input channels passes every shape check; CPU tensors then fail the device check
+    with pytest.raises(ValueError, match="HIP device"):
+        check_conv_args(torch.zeros(5, 3), w, 5, False, torch.zeros(5, 3))
+    with pytest.raises(ValueError, match="concatenated features"):
+        check_conv_args(torch.zeros(5, 3), w, 5, False, torch.zeros(5, 2))  # halves of different widths
+    with pytest.raises(ValueError, match="concatenated features"):
+        check_conv_args(torch.zeros(5, 3), w, 5, False, torch.zeros(5, 3, dtype=torch.float64))
+    with pytest.raises(ValueError, match="input channels"):
+        check_conv_args(torch.zeros(5, 3), w, 5, False)  # 3 channels against a 6-channel kernel
 
 
 def test_package_surface():
