@@ -365,6 +365,190 @@ __global__ __launch_bounds__(256) void k_wgrad_pipe(const T* __restrict__ X, int
             }
 }
 
+// Ring stages of k_wgrad_dma: four rounds of WGD_ROWS pairs x 128 channels for each
+// operand, 8 KB per stage and operand, as separate LDS objects so that the compiler tells
+// the stage being read from the stages being filled and does not wait for their copies.
+constexpr int WGD_ROWS = 32;
+static __shared__ __attribute__((aligned(16))) uint4 wgd_x0[512], wgd_x1[512], wgd_x2[512], wgd_x3[512];
+static __shared__ __attribute__((aligned(16))) uint4 wgd_g0[512], wgd_g1[512], wgd_g2[512], wgd_g3[512];
+// Source of the copies of pairs past a partial's end: one row of 128 zero channels (device
+// memory, zero-initialised at load, never written).
+static __device__ __attribute__((aligned(16))) uint4 wgd_zero_row[16];
+
+// Direct-to-LDS form of k_wgrad_pipe for 16-bit layers whose channel counts are
+// multiples of 128 (128 x 128 tiles): the rows of each 32-pair round are copied by the
+// load unit (global_load_lds_dwordx4) into a ring of four LDS stages, three rounds ahead
+// of the round being contracted, instead of passing through registers one round ahead.
+// Two workgroups per CU (64 KB of LDS each) then keep six 16 KB rounds in flight per CU
+// instead of two 32 KB ones; the rows come mostly from the last-level cache, whose
+// latency sets the rate (planes + spheres, 1.1M rows, 6.7M pairs, 128 -> 128 bf16 on
+// MI355X, A/B in one process: 0.477-0.480 ms against 0.492-0.510 ms for k_wgrad_pipe;
+// four 64-pair stages at one workgroup per CU ran 0.527 ms).
+// Offset k's pairs are cut into rounds of WGD_ROWS pairs taken interleaved by its
+// partials, as in k_wgrad_pipe with rounds of 32.  Every round issues the same copies: a
+// pair past the partial's end copies a row of zeros.  Pair indices are loaded two rounds
+// before their rows, so the compiler's wait for them (and the wait for a stage before it
+// is read) counts exactly the loads issued after.  Swizzled tiles and transpose reads as
+// k_wgrad_pipe.
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_wgrad_dma(const T* __restrict__ X, int cin, const T* __restrict__ G, int cout, const int32_t* __restrict__ pairs,
+                 const int64_t* __restrict__ k_offsets, const int32_t* __restrict__ chunk_start, int K, int transposed,
+                 float* __restrict__ ws) {
+    static_assert(sizeof(T) == 2, "16-bit operands");
+    using F = Frag<T>;
+    using V = typename F::vec;
+    constexpr int MI = 4, NI = 4, TM = 128, TN = 128, SEG = 16, RP = WGD_ROWS;
+    constexpr int PT = RP * SEG / 256;  // copies per thread and operand
+
+    const int id = blockIdx.x;
+    int lo = 0, hi = K;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (chunk_start[mid] <= id) lo = mid;
+        else hi = mid;
+    }
+    const int k = lo;
+    const int64_t stride = (int64_t)(chunk_start[k + 1] - chunk_start[k]) * RP;
+    const int64_t p0 = k_offsets[k] + (int64_t)(id - chunk_start[k]) * RP;
+    const int64_t p1 = k_offsets[k + 1];
+
+    const int n_ct = cin / TM;
+    const int c0 = (blockIdx.y % n_ct) * TM;
+    const int o0 = (blockIdx.y / n_ct) * TN;
+    const int ci = transposed ? 1 : 0, co = 1 - ci;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int wm = (wave & 1) * MI * 16, wn = (wave >> 1) * NI * 16;
+
+    // copy q of this thread: stage element e = tid + 256 q is row e / 16 at 16-byte
+    // position e % 16, which holds the row's segment (e % 16) ^ swizzle(row)
+    int rr[PT], so[PT];
+#pragma unroll
+    for (int q = 0; q < PT; ++q) {
+        const int e = tid + 256 * q, r = e / SEG;
+        rr[q] = r;
+        so[q] = ((e % SEG) ^ wg_swizzle<SEG>(r)) * 8;  // element offset of the segment this lane copies
+    }
+    const T* const zrow = reinterpret_cast<const T*>(wgd_zero_row);
+    floatx4 acc[MI][NI];
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NI; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    auto sx = [&](auto sc) -> T* {
+        constexpr int s = decltype(sc)::value & 3;
+        return reinterpret_cast<T*>(s == 0 ? wgd_x0 : s == 1 ? wgd_x1 : s == 2 ? wgd_x2 : wgd_x3);
+    };
+    auto sg = [&](auto sc) -> T* {
+        constexpr int s = decltype(sc)::value & 3;
+        return reinterpret_cast<T*>(s == 0 ? wgd_g0 : s == 1 ? wgd_g1 : s == 2 ? wgd_g2 : wgd_g3);
+    };
+    int32_t xi[2][PT], gi[2][PT];  // pair indices, two rounds in flight
+    auto load_idx = [&](int64_t j, int32_t (&xx)[PT], int32_t (&gg)[PT]) __attribute__((always_inline)) {
+        const int64_t p = p0 + j * stride;
+#pragma unroll
+        for (int q = 0; q < PT; ++q) {
+            const int64_t pp = p + rr[q] < p1 ? p + rr[q] : p1 - 1;
+            xx[q] = pairs[2 * pp + ci];
+            gg[q] = pairs[2 * pp + co];
+        }
+    };
+    // rows of round j into stage sc (the lanes of copy q of wavefront w fill 1 KB from
+    // element 256 q + 64 w on)
+    auto load_rows = [&](int64_t j, const int32_t (&xx)[PT], const int32_t (&gg)[PT], auto sc)
+                         __attribute__((always_inline)) {
+        const int64_t p = p0 + j * stride;
+        T* const dx = sx(sc);
+        T* const dg = sg(sc);
+#pragma unroll
+        for (int q = 0; q < PT; ++q) {
+            const bool ok = p + rr[q] < p1;
+            const T* const srcx = ok ? X + (int64_t)xx[q] * cin + c0 + so[q] : zrow + so[q];
+            const T* const srcg = ok ? G + (int64_t)gg[q] * cout + o0 + so[q] : zrow + so[q];
+            const int e0 = 256 * q + 64 * wave;
+            lds_dma16(srcx, dx + e0 * 8);
+            lds_dma16(srcg, dg + e0 * 8);
+        }
+    };
+    auto contract = [&](auto sc) __attribute__((always_inline)) {
+        const T* const sX = sx(sc);
+        const T* const sG = sg(sc);
+#pragma unroll
+        for (int st = 0; st < RP / F::CH; ++st) {
+            V a[MI], b[NI];
+            const int tr = st * F::CH + 8 * g + (r16 >> 2);
+            const int half = (r16 & 1) * 4;
+            const int sw = wg_swizzle<SEG>(tr);
+#pragma unroll
+            for (int i = 0; i < MI; ++i) {
+                const int seg = ((wm + i * 16) / 8 + ((r16 & 3) >> 1)) ^ sw;
+                a[i] = lds_frag_tr16<V>(&sX[tr * TM + seg * 8 + half], TM);
+            }
+#pragma unroll
+            for (int j = 0; j < NI; ++j) {
+                const int seg = ((wn + j * 16) / 8 + ((r16 & 3) >> 1)) ^ sw;
+                b[j] = lds_frag_tr16<V>(&sG[tr * TN + seg * 8 + half], TN);
+            }
+#pragma unroll
+            for (int i = 0; i < MI; ++i)
+#pragma unroll
+                for (int j = 0; j < NI; ++j) mfma(a[i], b[j], acc[i][j]);
+        }
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    const int64_t n_rounds = p0 < p1 ? (p1 - p0 + stride - 1) / stride : 0;
+    if (n_rounds > 0) {
+        // issue order: idx 0, idx 1, rows 0, idx 2, rows 1, idx 3, rows 2; then round j issues
+        // idx j+4 and rows j+3, so that when round j waits for rows j the loads after them are
+        // always idx j+2, rows j+1, idx j+3, rows j+2 (2 PT loads each)
+        load_idx(0, xi[0], gi[0]);
+        load_idx(1, xi[1], gi[1]);
+        load_rows(0, xi[0], gi[0], I0{});
+        load_idx(2, xi[0], gi[0]);
+        load_rows(1, xi[1], gi[1], I1{});
+        load_idx(3, xi[1], gi[1]);
+        load_rows(2, xi[0], gi[0], I2{});
+        // round j (u = j mod 4 fixes the stage and the index sets)
+        auto round = [&](int64_t j, auto uc) __attribute__((always_inline)) {
+            constexpr int u = decltype(uc)::value;
+            __builtin_amdgcn_sched_barrier(0);
+            wait_vm_lds<4 * 2 * PT>();  // rows j landed (idx j+2, rows j+1, idx j+3, rows j+2 may be in flight)
+            __syncthreads();            // ... in every wavefront, and stage j+3 = j-1 is no longer read
+            load_idx(j + 4, xi[u & 1], gi[u & 1]);
+            load_rows(j + 3, xi[(u + 1) & 1], gi[(u + 1) & 1], std::integral_constant<int, u + 3>{});
+            contract(uc);
+        };
+        int64_t j = 0;
+#pragma unroll 1
+        for (; j + 4 <= n_rounds; j += 4) {
+            round(j, I0{});
+            round(j + 1, I1{});
+            round(j + 2, I2{});
+            round(j + 3, std::integral_constant<int, 3>{});
+        }
+        if (j < n_rounds) round(j, I0{});
+        if (j + 1 < n_rounds) round(j + 1, I1{});
+        if (j + 2 < n_rounds) round(j + 2, I2{});
+        wait_vm_lds<0>();  // the copies issued past the last round land before the workgroup ends
+    }
+
+    float* __restrict__ out = ws + (int64_t)id * cin * cout;
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NI; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int c = c0 + wm + i * 16 + 4 * g + e;
+                const int o = o0 + wn + j * 16 + r16;
+                out[(int64_t)c * cout + o] = acc[i][j][e];
+            }
+}
+
 // fp64 (and any dtype) scalar weight gradient: one thread per (chunk, c, o).
 template <typename T, typename Acc>
 __global__ void k_wgrad_generic(const T* __restrict__ X, int cin, const T* __restrict__ G, int cout,
@@ -429,6 +613,13 @@ static void launch_wgrad_mfma(const T* X, int cin, const T* G, int cout, const i
     const dim3 grid((unsigned)total_chunks, (unsigned)(((cin + TM - 1) / TM) * ((cout + TN - 1) / TN)));
     const bool vec = cin % Frag<T>::EPL == 0 && cout % Frag<T>::EPL == 0 && (uintptr_t)X % 16 == 0 &&
                      (uintptr_t)G % 16 == 0;
+    if constexpr (sizeof(T) == 2 && MI == 4 && NI == 4) {
+        if (vec && cin % 128 == 0 && cout % 128 == 0) {
+            hipLaunchKernelGGL((k_wgrad_dma<T>), grid, dim3(256), 0, s, X, cin, G, cout, pairs, koff, cstart, K,
+                               transposed, ws);
+            return;
+        }
+    }
     if (vec)
         hipLaunchKernelGGL((k_wgrad_pipe<T, MI, NI, true>), grid, dim3(256), 0, s, X, cin, G, cout, pairs, koff, cstart,
                            K, transposed, ws);

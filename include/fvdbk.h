@@ -316,8 +316,9 @@ int fvdbk_kmap_row_order_windowed(const int32_t* map, int64_t n_rows, int32_t K,
  * (ordered by k, then output row).  Split-K: offset k has n_k = chunk_start[k+1] -
  * chunk_start[k] partials (chunk_start[k] = first partial of offset k, chunk_start[K] =
  * total_chunks, a device int32 array; the callers take n_k = ceil(pairs of k / chunk)).
- * The offset's pairs are cut into rounds of 64 and partial s sums rounds s, s + n_k,
- * s + 2 n_k, ..., so the partials of all offsets sweep the output rows together.  Each
+ * The offset's pairs are cut into rounds of 64 (32 for 16-bit dtypes with c_in and c_out
+ * multiples of 128) and partial s sums rounds s, s + n_k, s + 2 n_k, ..., so the partials
+ * of all offsets sweep the output rows together.  Each
  * partial's fp32 product goes to workspace [total_chunks][c_in][c_out] and a second pass
  * adds the partials of each offset with Neumaier-compensated fp32 summation (fp64 plain
  * sums for FVDBK_F64), in partial order, so the result is deterministic.  k_offsets [K+1] (device int64)
