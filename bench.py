@@ -206,10 +206,12 @@ def main() -> int:
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only loop the dominant kernel (for profiler runs)")
-    ap.add_argument("--workload", default="precision", choices=["precision", "single", "diff", "unet", "serve", "orders"],
+    ap.add_argument("--workload", default="precision",
+                    choices=["precision", "single", "diff", "unet", "serve", "orders", "rows"],
                     help="precision (default, the headline line); single / diff / unet / serve: the other "
                          "workloads, one JSON line each; orders: contraction per layer in tree order vs "
-                         "tap-mask order per window")
+                         "tap-mask order per window; rows: grid build, pooling, upsampling, injection, "
+                         "GroupNorm, queries, dense I/O and save / load on the sphere batch")
     ap.add_argument("--requests", type=int, default=120, help="serve: number of requests")
     ap.add_argument("--serve-requests", type=int, default=40,
                     help="requests of the short serving stream reported in the default line (0: none)")
@@ -230,7 +232,7 @@ def main() -> int:
     torch.cuda.set_device(dev)
     if args.workload != "precision":
         rec = {"single": run_single, "diff": run_diff, "unet": run_unet, "serve": run_serve,
-               "orders": run_orders}[args.workload](
+               "orders": run_orders, "rows": run_rows}[args.workload](
             torch, fvdb, dev, args)
         print(json.dumps(rec), flush=True)
         return 0
@@ -554,6 +556,98 @@ def run_diff(torch, fvdb, dev, args):
     return {"workload": "diff: planes+spheres at 1/1024, 3x3x3 128->128, forward + dgrad + wgrad", "voxels": N,
             "pairs": P, "flops_per_pass": F, "timings": out, "errors_vs_fp64_cpu_oracle": errs,
             "fd_check": {"subgrid_voxels": n, "channels": 8, "h": h, "max_rel_err": fd_err, "pass": fd_err < 1e-6}}
+
+
+def run_rows(torch, fvdb, dev, args):
+    """The operations either side of the contraction on the sphere batch (2.34M voxels, 64 bf16
+    channels): grid build, 2x coarsening, max / avg pooling, nearest upsampling, injection,
+    GroupNorm, voxel queries, dense read / write (one smaller grid) and save / load.  Each is the
+    median of single calls through the public API (HIP events around the call, host work inside it
+    included), set against its algorithmic HBM bytes (the minimum its inputs and outputs imply)."""
+    import os
+    import tempfile
+
+    from fvdb.utils.synthetic import sphere_points, voxelize
+
+    grid, vs, x, wa, wb = build_workload(torch, fvdb, dev)
+    del wa, wb
+    N, C, b = grid.total_voxels, 64, 2
+    xb = x.to(torch.bfloat16)
+    del x
+    xj = grid.jagged_like(xb)
+    res = {}
+
+    def grid_bytes(g):
+        return sum(t.numel() * t.element_size() for t in g._impl.tensors() if torch.is_tensor(t))
+
+    def put(name, ms, nbytes, rows):
+        gbps = nbytes / (ms * 1e-3) / 1e9
+        res[name] = {"ms": ms, "algorithmic_bytes": int(nbytes), "GBps": gbps, "hbm_frac": gbps / 8000.0,
+                     "rows_per_s": rows / (ms * 1e-3)}
+        log(f"[bench] rows {name}: {res[name]}")
+
+    with torch.no_grad():
+        ijk = grid.ijk
+        t = _median_ms(torch, lambda: fvdb.GridBatch.from_ijk(ijk, voxel_sizes=grid.voxel_sizes,
+                                                                origins=grid.origins), 10)
+        put("grid_from_ijk", t, N * 12 + grid_bytes(grid), N)
+        coarse = grid.coarsened_grid(2)
+        Nc = coarse.total_voxels
+        L = int(grid._impl.keys.shape[0])
+        t = _median_ms(torch, lambda: grid.coarsened_grid(2), 10)
+        put("coarsened_grid_2x", t, L * (8 + 64) + grid_bytes(coarse), Nc)
+        pool_bytes = N * C * b + Nc * C * b + 2 * Nc * 8 * 4  # rows in, windows out, window map written + read
+        t = _median_ms(torch, lambda: grid.max_pool(2, xj, coarse_grid=coarse), 10)
+        put("max_pool_2x", t, pool_bytes, Nc)
+        t = _median_ms(torch, lambda: grid.avg_pool(2, xj, coarse_grid=coarse), 10)
+        put("avg_pool_2x", t, pool_bytes, Nc)
+        cj = coarse.jagged_like(torch.randn(Nc, C, device=dev).to(torch.bfloat16))
+        t = _median_ms(torch, lambda: coarse.subdivide(2, cj, fine_grid=grid), 10)
+        put("upsample_nearest_2x", t, Nc * C * b + N * C * b + N * (12 + 4), N)
+        t = _median_ms(torch, lambda: grid.inject_from(grid, xj), 10)
+        put("inject_from", t, 3 * N * C * b + N * (12 + 4), N)
+        from fvdb._norm import group_norm
+
+        gw = torch.ones(C, device=dev, dtype=torch.bfloat16)
+        gb = torch.zeros(C, device=dev, dtype=torch.bfloat16)
+        t = _median_ms(torch, lambda: group_norm(xb, grid, 32, gw, gb), 10)
+        put("group_norm_32", t, 3 * N * C * b, N)
+        t = _median_ms(torch, lambda: grid.ijk_to_index(ijk), 10)
+        put("ijk_to_index", t, N * (12 + 8), N)
+        # dense I/O on one grid whose box stays small (8 channels)
+        gd = voxelize(sphere_points(seed=5, n_points=500_000, device=dev), 1 / 128)
+        Nd, Cd = gd.total_voxels, 8
+        xd = gd.jagged_like(torch.randn(Nd, Cd, device=dev).to(torch.bfloat16))
+        mc = gd.ijk.jdata.min(dim=0).values
+        dense = gd.write_to_dense_czyx(xd, min_coord=mc)
+        t = _median_ms(torch, lambda: gd.write_to_dense_czyx(xd, min_coord=mc), 10)
+        put("write_to_dense_czyx", t, dense.numel() * b + 2 * Nd * Cd * b + Nd * 12, Nd)
+        back = gd.read_from_dense_czyx(dense, dense_origins=mc)
+        res["dense_round_trip_equal"] = bool(torch.equal(back.jdata, xd.jdata))
+        t = _median_ms(torch, lambda: gd.read_from_dense_czyx(dense, dense_origins=mc), 10)
+        put("read_from_dense_czyx", t, 2 * Nd * Cd * b + Nd * 12, Nd)
+        res["dense_box"] = list(dense.shape)
+        del dense
+        # save / load through the host (file in the temporary directory)
+        fd, path = tempfile.mkstemp(suffix=".npz")
+        os.close(fd)
+        try:
+            t0 = time.perf_counter()
+            fvdb.save_gridbatch(path, grid, data=xj)
+            t_save = time.perf_counter() - t0
+            size = os.path.getsize(path)
+            t0 = time.perf_counter()
+            g2, d2, _ = fvdb.load_gridbatch(path, device=dev)
+            torch.cuda.synchronize()
+            t_load = time.perf_counter() - t0
+            same = bool(torch.equal(g2.ijk.jdata, grid.ijk.jdata) and torch.equal(d2.jdata, xb))
+        finally:
+            os.remove(path)
+        res["save_load"] = {"file_bytes": size, "save_ms": t_save * 1e3, "load_ms": t_load * 1e3,
+                            "save_MBps": size / t_save / 1e6, "load_MBps": size / t_load / 1e6, "round_trip_equal": same}
+        log(f"[bench] rows save_load: {res['save_load']}")
+    return {"workload": "rows: operations either side of the contraction on the 8-grid sphere batch (64 bf16 channels)",
+            "voxels": N, "coarse_voxels": Nc, "leaves": L, "dense_grid_voxels": Nd, "results": res}
 
 
 def run_orders(torch, fvdb, dev, args):

@@ -305,7 +305,7 @@ at::Tensor window_map(std::vector<at::Tensor> rows, std::vector<at::Tensor> prob
 }
 
 // Max pooling over a window map: (out [n_rows, C], argmax [n_rows, C] int32).
-std::vector<at::Tensor> pool_max(at::Tensor in, at::Tensor map, int64_t n_rows) {
+std::vector<at::Tensor> pool_max(at::Tensor in, at::Tensor map, int64_t n_rows, bool need_argmax) {
     require_hip(in, "features");
     TORCH_CHECK_VALUE(in.dim() == 2, "features must have shape [N, C]");
     TORCH_CHECK_VALUE(map.dim() == 2 && map.scalar_type() == at::kInt && map.size(0) >= n_rows,
@@ -314,10 +314,12 @@ std::vector<at::Tensor> pool_max(at::Tensor in, at::Tensor map, int64_t n_rows) 
     map = map.contiguous();
     c10::hip::HIPGuard guard(in.device().index());
     at::Tensor out = at::empty({n_rows, in.size(1)}, in.options());
-    at::Tensor arg = at::empty({n_rows, in.size(1)}, in.options().dtype(at::kInt));
+    // the argmax (the voxel that receives the gradient) only when a backward will need it
+    at::Tensor arg = at::empty({need_argmax ? n_rows : 0, in.size(1)}, in.options().dtype(at::kInt));
     if (n_rows > 0 && in.size(1) > 0)
         check_rc(fvdbk_pool_max(to_dtype(in.scalar_type()), in.data_ptr(), (int32_t)in.size(1), map.data_ptr<int32_t>(),
-                                n_rows, (int32_t)map.size(1), out.data_ptr(), arg.data_ptr<int32_t>(), stream_of(in)),
+                                n_rows, (int32_t)map.size(1), out.data_ptr(),
+                                need_argmax ? arg.data_ptr<int32_t>() : nullptr, stream_of(in)),
                  "pool_max");
     return {out, arg};
 }
@@ -336,6 +338,34 @@ at::Tensor pool_avg(at::Tensor in, at::Tensor map, int64_t n_rows, double scale)
         check_rc(fvdbk_pool_avg(to_dtype(in.scalar_type()), in.data_ptr(), (int32_t)in.size(1), map.data_ptr<int32_t>(),
                                 n_rows, (int32_t)map.size(1), scale, out.data_ptr(), stream_of(in)),
                  "pool_avg");
+    return out;
+}
+
+// out[i] = src[idx[i]] where idx[i] is a row of src, else fill[i] (zeros without fill); out has
+// idx.size(0) rows shaped like src's.
+at::Tensor gather_rows(at::Tensor src, at::Tensor idx, c10::optional<at::Tensor> fill) {
+    require_hip(src, "source rows");
+    TORCH_CHECK_VALUE(src.dim() >= 1, "source rows must have at least one dimension");
+    TORCH_CHECK_VALUE(idx.dim() == 1 && idx.scalar_type() == at::kInt && idx.device() == src.device(),
+                      "indices must be int32 [n] on the rows' device");
+    src = src.contiguous();
+    idx = idx.contiguous();
+    std::vector<int64_t> shape(src.sizes().begin(), src.sizes().end());
+    shape[0] = idx.size(0);
+    if (fill.has_value()) {
+        TORCH_CHECK_VALUE(fill->sizes() == at::IntArrayRef(shape) && fill->scalar_type() == src.scalar_type() &&
+                              fill->device() == src.device(),
+                          "fill must have the output's shape, dtype and device");
+        *fill = fill->contiguous();
+    }
+    c10::hip::HIPGuard guard(src.device().index());
+    at::Tensor out = at::empty(shape, src.options());
+    int64_t row_bytes = src.element_size();
+    for (int64_t d = 1; d < src.dim(); ++d) row_bytes *= src.size(d);
+    if (idx.size(0) > 0 && row_bytes > 0)
+        check_rc(fvdbk_gather_rows(src.data_ptr(), src.size(0), row_bytes, idx.data_ptr<int32_t>(), idx.size(0),
+                                   fill.has_value() ? fill->data_ptr() : nullptr, out.data_ptr(), stream_of(src)),
+                 "gather_rows");
     return out;
 }
 
@@ -650,8 +680,10 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("kernel_map", &kernel_map, nogil);
     m.def("neighbor_map", &neighbor_map, nogil);
     m.def("window_map", &window_map, nogil);
-    m.def("pool_max", &pool_max, nogil);
+    m.def("pool_max", &pool_max, nogil, py::arg("in"), py::arg("map"), py::arg("n_rows"),
+          py::arg("need_argmax") = true);
     m.def("pool_avg", &pool_avg, nogil);
+    m.def("gather_rows", &gather_rows, nogil, py::arg("src"), py::arg("idx"), py::arg("fill") = py::none());
     m.def("implicit_gemm", &implicit_gemm, nogil, pybind11::arg("in"), pybind11::arg("a"), pybind11::arg("map"),
           pybind11::arg("n_rows"), pybind11::arg("order") = pybind11::none(), pybind11::arg("bias") = pybind11::none(),
           pybind11::arg("relu") = false);

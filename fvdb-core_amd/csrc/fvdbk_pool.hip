@@ -56,11 +56,85 @@ __global__ void k_pool_avg(const T* __restrict__ in, int C, const int32_t* __res
     out[idx] = (T)((A)(T)sum * (A)scale);
 }
 
+// Vector form of k_pool_max / k_pool_avg: one thread per (row, 16-byte channel segment), for
+// rows of whole 16-byte segments on 16-byte aligned arrays.  Each existing window voxel's
+// segment is one 16-byte load; taps, comparisons and arithmetic as the scalar kernels, so
+// the results are identical.
+template <typename T>
+union Seg16 {
+    uint4 v;
+    T e[16 / sizeof(T)];
+};
+
+template <typename T, bool MAX>
+__global__ __launch_bounds__(256) void k_pool_vec(const T* __restrict__ in, int segs, const int32_t* __restrict__ map,
+                                                  uint32_t total, int K, T scale, T* __restrict__ out,
+                                                  int32_t* __restrict__ argmax) {
+    using A = typename PoolAcc<T>::type;
+    constexpr int E = 16 / (int)sizeof(T);
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const uint32_t r = t / (uint32_t)segs, sg = t - r * (uint32_t)segs;
+    const int64_t row_elems = (int64_t)segs * E;
+    A acc[E];
+    int32_t arg[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        acc[e] = MAX ? (A)-INFINITY : (A)0;
+        arg[e] = -1;
+    }
+    const int32_t* __restrict__ m = map + (int64_t)r * K;
+    for (int k = 0; k < K; ++k) {
+        const int32_t s = m[k];
+        if (s < 0) continue;
+        Seg16<T> v;
+        v.v = *reinterpret_cast<const uint4*>(in + (int64_t)s * row_elems + (int64_t)sg * E);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const A x = (A)v.e[e];
+            if constexpr (MAX) {
+                if (x > acc[e]) {
+                    acc[e] = x;
+                    arg[e] = s;
+                }
+            } else {
+                acc[e] += x;
+            }
+        }
+    }
+    Seg16<T> o;
+#pragma unroll
+    for (int e = 0; e < E; ++e) o.e[e] = MAX ? (T)acc[e] : (T)((A)(T)acc[e] * (A)scale);
+    const int64_t base = (int64_t)r * row_elems + (int64_t)sg * E;
+    *reinterpret_cast<uint4*>(out + base) = o.v;
+    if (MAX && argmax != nullptr) {
+#pragma unroll
+        for (int q = 0; q < E / 4; ++q)
+            *reinterpret_cast<int4*>(argmax + base + 4 * q) = make_int4(arg[4 * q], arg[4 * q + 1], arg[4 * q + 2],
+                                                                        arg[4 * q + 3]);
+        if constexpr (E == 2) *reinterpret_cast<int2*>(argmax + base) = make_int2(arg[0], arg[1]);
+    }
+}
+
 template <typename T>
 static int pool_launch(bool is_max, const void* in, int C, const int32_t* map, int64_t n_rows, int K, double scale,
                        void* out, int32_t* argmax, hipStream_t s) {
     const int64_t n = n_rows * C;
     if (n == 0) return FVDBK_OK;
+    constexpr int E = 16 / (int)sizeof(T);
+    const bool vec = C % E == 0 && n / E < ((int64_t)1 << 31) &&
+                     (((uintptr_t)in | (uintptr_t)out | (uintptr_t)argmax) & 15u) == 0;
+    if (vec) {
+        const uint32_t total = (uint32_t)(n / E);
+        if (is_max)
+            hipLaunchKernelGGL((k_pool_vec<T, true>), dim3(grid_1d(total, 256)), dim3(256), 0, s, (const T*)in, C / E,
+                               map, total, K, (T)0, (T*)out, argmax);
+        else
+            hipLaunchKernelGGL((k_pool_vec<T, false>), dim3(grid_1d(total, 256)), dim3(256), 0, s, (const T*)in,
+                               C / E, map, total, K, (T)scale, (T*)out, nullptr);
+        FVDBK_LAUNCH_CHECK("k_pool_vec");
+        return FVDBK_OK;
+    }
     if (is_max) {
         hipLaunchKernelGGL(k_pool_max<T>, dim3(grid_1d(n, 256)), dim3(256), 0, s, (const T*)in, C, map, n_rows, K,
                            (T*)out, argmax);
@@ -87,6 +161,23 @@ static int pool_entry(bool is_max, fvdbk_dtype dtype, const void* in, int32_t C,
     }
 }
 
+// out[i] = src[idx[i]] where 0 <= idx[i] < n_src, else fill[i] (zeros when fill is null),
+// one thread per (row, U-sized unit of the row).
+template <typename U>
+__global__ __launch_bounds__(256) void k_gather_rows(const U* __restrict__ src, int64_t n_src, int64_t units,
+                                                     const int32_t* __restrict__ idx, int64_t total,
+                                                     const U* __restrict__ fill, U* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int64_t r = t / units, u = t - r * units;
+    const int32_t s = idx[r];
+    U v;
+    if (s >= 0 && s < n_src) v = src[(int64_t)s * units + u];
+    else if (fill != nullptr) v = fill[t];
+    else v = U{};
+    out[t] = v;
+}
+
 }  // namespace fvdbk
 
 using namespace fvdbk;
@@ -101,6 +192,25 @@ int fvdbk_pool_max(fvdbk_dtype dtype, const void* in, int32_t channels, const in
 int fvdbk_pool_avg(fvdbk_dtype dtype, const void* in, int32_t channels, const int32_t* map, int64_t n_rows, int32_t K,
                    double scale, void* out, hipStream_t stream) {
     return pool_entry(false, dtype, in, channels, map, n_rows, K, scale, out, nullptr, stream);
+}
+
+int fvdbk_gather_rows(const void* src, int64_t n_src, int64_t row_bytes, const int32_t* idx, int64_t n,
+                      const void* fill, void* out, hipStream_t stream) {
+    FVDBK_REQUIRE(row_bytes > 0 && n >= 0 && n_src >= 0, FVDBK_EINVAL, "row gather: bad sizes");
+    if (n == 0) return FVDBK_OK;
+    FVDBK_REQUIRE(idx && out && (src || n_src == 0), FVDBK_EINVAL, "row gather: null argument");
+    const uintptr_t al = (uintptr_t)src | (uintptr_t)out | (uintptr_t)fill;
+    auto go = [&](auto unit) {
+        using U = decltype(unit);
+        const int64_t units = row_bytes / (int64_t)sizeof(U), total = n * units;
+        hipLaunchKernelGGL((k_gather_rows<U>), dim3(grid_1d(total, 256)), dim3(256), 0, stream, (const U*)src, n_src,
+                           units, idx, total, (const U*)fill, (U*)out);
+    };
+    if (row_bytes % 16 == 0 && (al & 15u) == 0) go(uint4{});
+    else if (row_bytes % 4 == 0 && (al & 3u) == 0) go(uint32_t{});
+    else go(uint8_t{});
+    FVDBK_LAUNCH_CHECK("k_gather_rows");
+    return FVDBK_OK;
 }
 
 }  // extern "C"

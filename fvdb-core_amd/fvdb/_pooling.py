@@ -46,7 +46,8 @@ class MaxPoolFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, data, wmap, n_coarse):
         x = _flat(data)
-        out, arg = ext.pool_max(x, wmap, n_coarse)
+        # the argmax (the voxel each coarse gradient goes to) only when a backward will run
+        out, arg = ext.pool_max(x, wmap, n_coarse, bool(ctx.needs_input_grad[0]))
         ctx.save_for_backward(arg)
         ctx.in_shape = data.shape
         return out.reshape((n_coarse,) + tuple(data.shape[1:]))
@@ -90,9 +91,7 @@ class GatherRowsFunction(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, data, idx):
-        valid = idx >= 0
-        out = torch.zeros((idx.shape[0],) + tuple(data.shape[1:]), dtype=data.dtype, device=data.device)
-        out[valid] = data[idx[valid].long()]
+        out = ext.gather_rows(data, idx)  # one pass: rows of missing sources are zero
         ctx.save_for_backward(idx)
         ctx.n_src = data.shape[0]
         return out
@@ -111,9 +110,12 @@ class InjectFunction(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, src, dst, idx):
-        valid = idx >= 0
-        out = dst.clone()
-        out[valid] = src[idx[valid].long()]
+        if dst.dtype == src.dtype:
+            out = ext.gather_rows(src, idx, dst)  # one pass: destination rows without a source keep dst
+        else:
+            valid = idx >= 0
+            out = dst.clone()
+            out[valid] = src[idx[valid].long()]
         ctx.save_for_backward(idx)
         ctx.n_src = src.shape[0]
         return out

@@ -522,8 +522,11 @@ class GridBatch:
         fine = fine_grid if fine_grid is not None else self.subdivided_grid(f, mask)
         if fine.grid_count != self.grid_count:
             raise ValueError("fine_grid must hold as many grids as this grid batch")
-        ft = torch.tensor(f, dtype=torch.int32, device=self.device)
-        parent = torch.div(fine._impl.ijk, ft, rounding_mode="floor")
+        if f[0] == f[1] == f[2]:
+            parent = torch.div(fine._impl.ijk, f[0], rounding_mode="floor")
+        else:
+            parent = torch.div(fine._impl.ijk, torch.tensor(f, dtype=torch.int32, device=self.device),
+                               rounding_mode="floor")
         idx = ext.ijk_to_index(*self._impl.tensors(), self.grid_count, parent.contiguous(), fine._impl.jidx)
         return fine.jagged_like(GatherRowsFunction.apply(d, idx)), fine
 

@@ -225,6 +225,13 @@ int fvdbk_pool_max(fvdbk_dtype dtype, const void* in, int32_t channels, const in
 int fvdbk_pool_avg(fvdbk_dtype dtype, const void* in, int32_t channels, const int32_t* map, int64_t n_rows,
                    int32_t K, double scale, void* out, hipStream_t stream);
 
+/* Row gather with fill (replaces the per-voxel copies of src/fvdb/detail/ops/UpsampleGridNearest.cu:17-96
+ * and ops/Inject.cu): out[i] = src[idx[i]] where 0 <= idx[i] < n_src, else fill[i], or zeros when
+ * fill is NULL; rows of row_bytes bytes (any element type), n rows out.  out must not overlap src
+ * or fill.  Copies in 16-byte units when rows and pointers allow, else 4-byte, else bytes. */
+int fvdbk_gather_rows(const void* src, int64_t n_src, int64_t row_bytes, const int32_t* idx, int64_t n,
+                      const void* fill, void* out, hipStream_t stream);
+
 /* ---------------------------------------------------------------------------
  * Gather-scatter neighbour map (replaces SparseConvPackInfo::buildGatherScatter,
  * src/fvdb/SparseConvPackInfo.cpp:61-88): pairs (source index, target row)
