@@ -612,6 +612,18 @@ def run_rows(torch, fvdb, dev, args):
         gb = torch.zeros(C, device=dev, dtype=torch.bfloat16)
         t = _median_ms(torch, lambda: group_norm(xb, grid, 32, gw, gb), 10)
         put("group_norm_32", t, 3 * N * C * b, N)
+        xg = xb.detach().requires_grad_(True)
+        dyg = torch.randn_like(xb)
+
+        def gn_fwd_bwd():
+            xg.grad = None
+            with torch.enable_grad():
+                group_norm(xg, grid, 32, gw, gb).backward(dyg)
+
+        t = _median_ms(torch, gn_fwd_bwd, 10)
+        # forward 3 passes; backward reads x and dy twice and writes dx
+        put("group_norm_32_fwd_bwd", t, 8 * N * C * b, N)
+        del xg, dyg
         t = _median_ms(torch, lambda: grid.ijk_to_index(ijk), 10)
         put("ijk_to_index", t, N * (12 + 8), N)
         # dense I/O on one grid whose box stays small (8 channels)

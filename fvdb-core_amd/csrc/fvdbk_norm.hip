@@ -8,10 +8,12 @@
 // Every reduction has a fixed order, so results are deterministic and independent of the
 // other grids in the batch:
 //   1. a partial pass over blocks of GN_ROWS rows of one grid: per channel, Welford (count,
-//      mean, M2) of the block's rows, threads of a channel merged in thread order;
+//      mean, M2) of the rows each thread visits, the threads of a channel merged in thread
+//      order (pairwise, q with q + 1, q + 2, q + 4 ..., in the 16-byte vector form);
 //   2. one wavefront per (grid, group) merges its blocks' partials (lanes stride over
 //      (block, channel) items, then a fixed butterfly) with Chan's pairwise update;
-//   3. y = (x - mean) * rstd * gamma + beta, optionally ReLU, one thread per element.
+//   3. y = (x - mean) * rstd * gamma + beta, optionally ReLU, one thread per element (per
+//      16-byte channel segment of GN_RPT rows in the vector form).
 // The backward pass reduces sum(dy) and sum(dy * xhat) per (grid, channel) the same way and
 // forms dx = rstd * (gamma dy - mean_g(gamma dy) - xhat mean_g(gamma dy xhat)).
 // Accumulation is fp32 (fp64 for fp64 data).
@@ -161,6 +163,258 @@ __global__ void k_gn_apply(const T* __restrict__ x, int64_t n, int C, int G, con
     y[idx] = (T)v;
 }
 
+// ---- Vector forms: C a multiple of E = 16 / sizeof(T) and row data 16-byte aligned.  A thread
+// owns one 16-byte segment (E channels) of a row and walks rows; a block covers up to
+// GN_THREADS segments of a row (segment chunk s0 .. s0 + ss) and R = GN_THREADS / ss rows at a time.
+
+template <typename T> union GnSeg {
+    uint4 v;
+    T e[16 / sizeof(T)];
+};
+
+constexpr int GN_RPT = 8;  // rows per thread in the element-wise kernels
+
+// Pass 1 (vector form), same partition and output as k_gn_stats_partial: thread (q, s) runs a
+// Welford update over rows r0 + q, r0 + q + R, ... of segment s, one count and one reciprocal
+// per row shared by the E channels; the R partials of a segment then merge pairwise in LDS
+// (q with q + h for h = 1, 2, 4, ...).  The order depends only on the block's own rows.
+template <typename T>
+__global__ __launch_bounds__(GN_THREADS) void k_gn_stats_partial_vec(const T* __restrict__ x, int C,
+                                                                     const int64_t* __restrict__ off,
+                                                                     const int32_t* __restrict__ prefix, int B,
+                                                                     Welford<typename NormAcc<T>::type>* __restrict__ part) {
+    using A = typename NormAcc<T>::type;
+    constexpr int E = 16 / sizeof(T);
+    __shared__ A s_mean[GN_THREADS * E], s_m2[GN_THREADS * E], s_n[GN_THREADS];
+    const int blk = blockIdx.x, tid = threadIdx.x;
+    int b;
+    int64_t r0, r1;
+    gn_block_rows(off, prefix, B, blk, b, r0, r1);
+    const int S = C / E, nrows = (int)(r1 - r0);
+    for (int s0 = 0; s0 < S; s0 += GN_THREADS) {
+        const int ss = min(GN_THREADS, S - s0), R = GN_THREADS / ss, cc = ss * E;
+        const int q = tid / ss, s = tid % ss;
+        A n = (A)0, mu[E], m2[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) mu[e] = m2[e] = (A)0;
+        if (q < R) {
+            const uint4* xs = reinterpret_cast<const uint4*>(x + r0 * C) + s0 + s;
+            auto update = [&](const GnSeg<T>& v) {
+                n += (A)1;
+                const A inv = (A)1 / n;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const A xv = (A)v.e[e];
+                    const A d = xv - mu[e];
+                    mu[e] += d * inv;
+                    m2[e] += d * (xv - mu[e]);
+                }
+            };
+            int r = q;
+            for (; r + 3 * R < nrows; r += 4 * R) {
+                GnSeg<T> v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u].v = xs[(int64_t)(r + u * R) * S];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) update(v[u]);
+            }
+            for (; r < nrows; r += R) {
+                GnSeg<T> v;
+                v.v = xs[(int64_t)r * S];
+                update(v);
+            }
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                s_mean[q * cc + s * E + e] = mu[e];
+                s_m2[q * cc + s * E + e] = m2[e];
+            }
+            if (s == 0) s_n[q] = n;
+        }
+        for (int h = 1; h < R; h <<= 1) {
+            __syncthreads();
+            const int pairs = (R + 2 * h - 1) / (2 * h);
+            for (int it = tid; it < pairs * cc; it += GN_THREADS) {
+                const int qa = (it / cc) * 2 * h, qb = qa + h, c = it % cc;
+                if (qb >= R) continue;
+                const A na = s_n[qa], nb = s_n[qb];
+                if (nb == (A)0) continue;
+                if (na == (A)0) {
+                    s_mean[qa * cc + c] = s_mean[qb * cc + c];
+                    s_m2[qa * cc + c] = s_m2[qb * cc + c];
+                    continue;
+                }
+                const A f = nb / (na + nb);
+                const A d = s_mean[qb * cc + c] - s_mean[qa * cc + c];
+                s_mean[qa * cc + c] += d * f;
+                s_m2[qa * cc + c] += s_m2[qb * cc + c] + d * d * na * f;
+            }
+            __syncthreads();
+            if (tid < pairs && tid * 2 * h + h < R) s_n[tid * 2 * h] += s_n[tid * 2 * h + h];
+        }
+        __syncthreads();
+        for (int c = tid; c < cc; c += GN_THREADS)
+            part[(int64_t)blk * C + s0 * E + c] = Welford<A>{s_n[0], s_mean[c], s_m2[c]};
+        __syncthreads();
+    }
+}
+
+// Rows of thread (q, s) in the element-wise vector kernels: rb + q + k R for k < GN_RPT, with
+// rb = blockIdx.x * R * GN_RPT; segment s0 + s with s0 = blockIdx.y * GN_THREADS.  The grid of
+// the first row comes from jidx (all rows are grid 0 without it); later rows advance it along
+// the offsets, since grids are contiguous and in order.
+
+// Pass 3 (vector form): y = (x - mean) * rstd * gamma + beta, the same arithmetic as k_gn_apply.
+template <typename T>
+__global__ __launch_bounds__(GN_THREADS) void k_gn_apply_vec(const T* __restrict__ x, int64_t n, int C, int G,
+                                                             const int64_t* __restrict__ off,
+                                                             const int32_t* __restrict__ jidx,
+                                                             const typename NormAcc<T>::type* __restrict__ mean,
+                                                             const typename NormAcc<T>::type* __restrict__ rstd,
+                                                             const T* __restrict__ gamma, const T* __restrict__ beta,
+                                                             int relu, T* __restrict__ y) {
+    using A = typename NormAcc<T>::type;
+    constexpr int E = 16 / sizeof(T);
+    const int S = C / E, s0 = blockIdx.y * GN_THREADS;
+    const int ss = min(GN_THREADS, S - s0), R = GN_THREADS / ss;
+    const int q = threadIdx.x / ss, s = s0 + threadIdx.x % ss;
+    const int64_t rq = (int64_t)blockIdx.x * R * GN_RPT + q;
+    if (q >= R || rq >= n) return;
+    const int c0 = s * E, Cg = C / G;
+    A w[E], bt[E];
+    int grp[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        w[e] = gamma ? (A)gamma[c0 + e] : (A)1;
+        bt[e] = beta ? (A)beta[c0 + e] : (A)0;
+        grp[e] = (c0 + e) / Cg;
+    }
+    const uint4* xs = reinterpret_cast<const uint4*>(x) + s;
+    uint4* ys = reinterpret_cast<uint4*>(y) + s;
+    GnSeg<T> v[GN_RPT];
+#pragma unroll
+    for (int k = 0; k < GN_RPT; ++k)
+        if (rq + k * R < n) v[k].v = xs[(rq + k * R) * S];
+    int b = jidx ? jidx[rq] : 0;
+    int64_t bend = off[b + 1];
+    A mu[E], rs[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        mu[e] = mean[b * G + grp[e]];
+        rs[e] = rstd[b * G + grp[e]];
+    }
+#pragma unroll
+    for (int k = 0; k < GN_RPT; ++k) {
+        const int64_t r = rq + k * R;
+        if (r >= n) break;
+        if (r >= bend) {
+            do bend = off[++b + 1];
+            while (r >= bend);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                mu[e] = mean[b * G + grp[e]];
+                rs[e] = rstd[b * G + grp[e]];
+            }
+        }
+        GnSeg<T> o;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            A t = ((A)v[k].e[e] - mu[e]) * rs[e];
+            if (gamma) t = t * w[e];
+            if (beta) t = t + bt[e];
+            if (relu && !(t > (A)0) && t == t) t = (A)0;
+            o.e[e] = (T)t;
+        }
+        ys[r * S] = o.v;
+    }
+}
+
+// Backward pass 1 (vector form), same partition and output as k_gn_bwd_partial: per thread
+// sequential sums over its rows, then the R partials of a segment summed pairwise in LDS.
+template <typename T>
+__global__ __launch_bounds__(GN_THREADS) void k_gn_bwd_partial_vec(const T* __restrict__ x, const T* __restrict__ dy,
+                                                                   const T* __restrict__ y, int C, int G,
+                                                                   const int64_t* __restrict__ off,
+                                                                   const int32_t* __restrict__ prefix, int B,
+                                                                   const typename NormAcc<T>::type* __restrict__ mean,
+                                                                   const typename NormAcc<T>::type* __restrict__ rstd,
+                                                                   typename NormAcc<T>::type* __restrict__ part) {
+    using A = typename NormAcc<T>::type;
+    constexpr int E = 16 / sizeof(T);
+    __shared__ A s_1[GN_THREADS * E], s_2[GN_THREADS * E];
+    const int blk = blockIdx.x, tid = threadIdx.x;
+    int b;
+    int64_t r0, r1;
+    gn_block_rows(off, prefix, B, blk, b, r0, r1);
+    const int S = C / E, nrows = (int)(r1 - r0), Cg = C / G;
+    for (int s0 = 0; s0 < S; s0 += GN_THREADS) {
+        const int ss = min(GN_THREADS, S - s0), R = GN_THREADS / ss, cc = ss * E;
+        const int q = tid / ss, s = tid % ss;
+        if (q < R) {
+            A a1[E], a2[E], mu[E], rs[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                a1[e] = a2[e] = (A)0;
+                const int g = b * G + ((s0 + s) * E + e) / Cg;
+                mu[e] = mean[g];
+                rs[e] = rstd[g];
+            }
+            const int64_t base = r0 * S + s0 + s;
+            const uint4* xs = reinterpret_cast<const uint4*>(x) + base;
+            const uint4* ds = reinterpret_cast<const uint4*>(dy) + base;
+            const uint4* ps = y ? reinterpret_cast<const uint4*>(y) + base : nullptr;
+            auto update = [&](const GnSeg<T>& xv, const GnSeg<T>& dv, const GnSeg<T>& yv) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    A d = (A)dv.e[e];
+                    if (ps && !((A)yv.e[e] > (A)0)) d = (A)0;
+                    a1[e] += d;
+                    a2[e] += d * (((A)xv.e[e] - mu[e]) * rs[e]);
+                }
+            };
+            int r = q;
+            for (; r + 3 * R < nrows; r += 4 * R) {
+                GnSeg<T> xv[4], dv[4], yv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    xv[u].v = xs[(int64_t)(r + u * R) * S];
+                    dv[u].v = ds[(int64_t)(r + u * R) * S];
+                    yv[u].v = ps ? ps[(int64_t)(r + u * R) * S] : uint4{0, 0, 0, 0};
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) update(xv[u], dv[u], yv[u]);
+            }
+            for (; r < nrows; r += R) {
+                GnSeg<T> xv, dv, yv;
+                xv.v = xs[(int64_t)r * S];
+                dv.v = ds[(int64_t)r * S];
+                yv.v = ps ? ps[(int64_t)r * S] : uint4{0, 0, 0, 0};
+                update(xv, dv, yv);
+            }
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                s_1[q * cc + s * E + e] = a1[e];
+                s_2[q * cc + s * E + e] = a2[e];
+            }
+        }
+        for (int h = 1; h < R; h <<= 1) {
+            __syncthreads();
+            const int pairs = (R + 2 * h - 1) / (2 * h);
+            for (int it = tid; it < pairs * cc; it += GN_THREADS) {
+                const int qa = (it / cc) * 2 * h, qb = qa + h, c = it % cc;
+                if (qb >= R) continue;
+                s_1[qa * cc + c] += s_1[qb * cc + c];
+                s_2[qa * cc + c] += s_2[qb * cc + c];
+            }
+        }
+        __syncthreads();
+        for (int c = tid; c < cc; c += GN_THREADS) {
+            part[((int64_t)blk * C + s0 * E + c) * 2] = s_1[c];
+            part[((int64_t)blk * C + s0 * E + c) * 2 + 1] = s_2[c];
+        }
+        __syncthreads();
+    }
+}
+
 // Backward pass 1: per block and channel, sum(dy) and sum(dy * xhat) (dy masked by y > 0
 // after a fused ReLU) -> part[blk][C][2].
 template <typename T>
@@ -288,6 +542,98 @@ __global__ void k_gn_bwd_dx(const T* __restrict__ x, const T* __restrict__ dy, c
     dx[idx] = (T)(rstd[s] * (w * d - gsums[2 * s] / M - xh * gsums[2 * s + 1] / M));
 }
 
+// Backward pass 4 (vector form), the same arithmetic as k_gn_bwd_dx; rows as in k_gn_apply_vec.
+template <typename T>
+__global__ __launch_bounds__(GN_THREADS) void k_gn_bwd_dx_vec(const T* __restrict__ x, const T* __restrict__ dy,
+                                                              const T* __restrict__ y, int64_t n, int C, int G,
+                                                              const int64_t* __restrict__ off,
+                                                              const int32_t* __restrict__ jidx,
+                                                              const typename NormAcc<T>::type* __restrict__ mean,
+                                                              const typename NormAcc<T>::type* __restrict__ rstd,
+                                                              const T* __restrict__ gamma,
+                                                              const typename NormAcc<T>::type* __restrict__ gsums,
+                                                              T* __restrict__ dx) {
+    using A = typename NormAcc<T>::type;
+    constexpr int E = 16 / sizeof(T);
+    const int S = C / E, s0 = blockIdx.y * GN_THREADS;
+    const int ss = min(GN_THREADS, S - s0), R = GN_THREADS / ss;
+    const int q = threadIdx.x / ss, s = s0 + threadIdx.x % ss;
+    const int64_t rq = (int64_t)blockIdx.x * R * GN_RPT + q;
+    if (q >= R || rq >= n) return;
+    const int c0 = s * E, Cg = C / G;
+    A w[E];
+    int grp[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        w[e] = gamma ? (A)gamma[c0 + e] : (A)1;
+        grp[e] = (c0 + e) / Cg;
+    }
+    const uint4* xs = reinterpret_cast<const uint4*>(x) + s;
+    const uint4* ds = reinterpret_cast<const uint4*>(dy) + s;
+    const uint4* ps = y ? reinterpret_cast<const uint4*>(y) + s : nullptr;
+    uint4* os = reinterpret_cast<uint4*>(dx) + s;
+    GnSeg<T> xv[GN_RPT], dv[GN_RPT];
+#pragma unroll
+    for (int k = 0; k < GN_RPT; ++k)
+        if (rq + k * R < n) {
+            xv[k].v = xs[(rq + k * R) * S];
+            dv[k].v = ds[(rq + k * R) * S];
+        }
+    int b = jidx ? jidx[rq] : 0;
+    int64_t bend = off[b + 1];
+    A mu[E], rs[E], t1[E], t2[E];
+    auto load_grid = [&]() {
+        const A M = (A)(off[b + 1] - off[b]) * (A)Cg;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int g = b * G + grp[e];
+            mu[e] = mean[g];
+            rs[e] = rstd[g];
+            t1[e] = gsums[2 * g] / M;
+            t2[e] = gsums[2 * g + 1] / M;
+        }
+    };
+    load_grid();
+#pragma unroll
+    for (int k = 0; k < GN_RPT; ++k) {
+        const int64_t r = rq + k * R;
+        if (r >= n) break;
+        if (r >= bend) {
+            do bend = off[++b + 1];
+            while (r >= bend);
+            load_grid();
+        }
+        GnSeg<T> yv, o;
+        if (ps) yv.v = ps[r * S];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            A d = (A)dv[k].e[e];
+            if (ps && !((A)yv.e[e] > (A)0)) d = (A)0;
+            const A xh = ((A)xv[k].e[e] - mu[e]) * rs[e];
+            o.e[e] = (T)(rs[e] * (w[e] * d - t1[e] - xh * t2[e]));
+        }
+        os[r * S] = o.v;
+    }
+}
+
+// The vector kernels apply when C is a multiple of 16 / sizeof(T) and every row array is
+// 16-byte aligned.
+template <typename T>
+static bool gn_vec_ok(int C, const void* a, const void* b, const void* c, const void* d) {
+    const uintptr_t m = (uintptr_t)a | (uintptr_t)b | (uintptr_t)c | (uintptr_t)d;
+    return C % (16 / (int)sizeof(T)) == 0 && (m & 15) == 0;
+}
+
+// Launch grid of the element-wise vector kernels: x over blocks of R * GN_RPT rows, y over
+// chunks of GN_THREADS segments (R of the first, widest chunk, so every chunk covers all rows).
+template <typename T>
+static dim3 gn_vec_grid(int64_t n, int C) {
+    const int S = C / (16 / (int)sizeof(T));
+    const int R = GN_THREADS / (S < GN_THREADS ? S : GN_THREADS);
+    return dim3((unsigned)((n + (int64_t)R * GN_RPT - 1) / ((int64_t)R * GN_RPT)),
+                (unsigned)((S + GN_THREADS - 1) / GN_THREADS));
+}
+
 template <typename T>
 static int gn_forward(const T* x, int64_t n, int C, int G, const int64_t* off, const int32_t* jidx, int B,
                       int64_t nb, const T* gamma, const T* beta, double eps, int relu, T* y, void* mean, void* rstd,
@@ -297,9 +643,14 @@ static int gn_forward(const T* x, int64_t n, int C, int G, const int64_t* off, c
     Welford<A>* part = reinterpret_cast<Welford<A>*>(static_cast<char*>(ws) + 256 * ((4 * (B + 1) + 255) / 256));
     hipLaunchKernelGGL(k_gn_block_prefix, dim3(1), dim3(1), 0, s, off, B, prefix);
     FVDBK_LAUNCH_CHECK("k_gn_block_prefix");
+    const bool vec = gn_vec_ok<T>(C, x, y, y, y);
     if (nb > 0) {
-        hipLaunchKernelGGL((k_gn_stats_partial<T>), dim3((unsigned)nb), dim3(GN_THREADS), 0, s, x, C, off, prefix, B,
-                           part);
+        if (vec)
+            hipLaunchKernelGGL((k_gn_stats_partial_vec<T>), dim3((unsigned)nb), dim3(GN_THREADS), 0, s, x, C, off,
+                               prefix, B, part);
+        else
+            hipLaunchKernelGGL((k_gn_stats_partial<T>), dim3((unsigned)nb), dim3(GN_THREADS), 0, s, x, C, off, prefix,
+                               B, part);
         FVDBK_LAUNCH_CHECK("k_gn_stats_partial");
     }
     const int64_t waves = (int64_t)B * G;
@@ -307,8 +658,12 @@ static int gn_forward(const T* x, int64_t n, int C, int G, const int64_t* off, c
                        eps, (A*)mean, (A*)rstd);
     FVDBK_LAUNCH_CHECK("k_gn_stats_final");
     if (n > 0) {
-        hipLaunchKernelGGL((k_gn_apply<T>), dim3(grid_1d(n * C, 256)), dim3(256), 0, s, x, n, C, G, jidx,
-                           (const A*)mean, (const A*)rstd, gamma, beta, relu, y);
+        if (vec)
+            hipLaunchKernelGGL((k_gn_apply_vec<T>), gn_vec_grid<T>(n, C), dim3(GN_THREADS), 0, s, x, n, C, G, off,
+                               jidx, (const A*)mean, (const A*)rstd, gamma, beta, relu, y);
+        else
+            hipLaunchKernelGGL((k_gn_apply<T>), dim3(grid_1d(n * C, 256)), dim3(256), 0, s, x, n, C, G, jidx,
+                               (const A*)mean, (const A*)rstd, gamma, beta, relu, y);
         FVDBK_LAUNCH_CHECK("k_gn_apply");
     }
     return FVDBK_OK;
@@ -329,9 +684,14 @@ static int gn_backward(const T* x, const T* dy, const T* y, int64_t n, int C, in
     A* gsums = reinterpret_cast<A*>(p);
     hipLaunchKernelGGL(k_gn_block_prefix, dim3(1), dim3(1), 0, s, off, B, prefix);
     FVDBK_LAUNCH_CHECK("k_gn_block_prefix");
+    const bool vec = gn_vec_ok<T>(C, x, dy, y ? y : x, dx);
     if (nb > 0) {
-        hipLaunchKernelGGL((k_gn_bwd_partial<T>), dim3((unsigned)nb), dim3(GN_THREADS), 0, s, x, dy, y, C, G, off,
-                           prefix, B, (const A*)mean, (const A*)rstd, part);
+        if (vec)
+            hipLaunchKernelGGL((k_gn_bwd_partial_vec<T>), dim3((unsigned)nb), dim3(GN_THREADS), 0, s, x, dy, y, C, G,
+                               off, prefix, B, (const A*)mean, (const A*)rstd, part);
+        else
+            hipLaunchKernelGGL((k_gn_bwd_partial<T>), dim3((unsigned)nb), dim3(GN_THREADS), 0, s, x, dy, y, C, G, off,
+                               prefix, B, (const A*)mean, (const A*)rstd, part);
         FVDBK_LAUNCH_CHECK("k_gn_bwd_partial");
     }
     hipLaunchKernelGGL((k_gn_bwd_grid_sums<A>), dim3(grid_1d((int64_t)B * C * 64, 256)), dim3(256), 0, s, part, C,
@@ -343,8 +703,12 @@ static int gn_backward(const T* x, const T* dy, const T* y, int64_t n, int C, in
                        dbl ? (double*)dgamma : nullptr, dbl ? (double*)dbeta : nullptr);
     FVDBK_LAUNCH_CHECK("k_gn_bwd_group_sums");
     if (n > 0) {
-        hipLaunchKernelGGL((k_gn_bwd_dx<T>), dim3(grid_1d(n * C, 256)), dim3(256), 0, s, x, dy, y, n, C, G, off, jidx,
-                           (const A*)mean, (const A*)rstd, gamma, gsums, dx);
+        if (vec)
+            hipLaunchKernelGGL((k_gn_bwd_dx_vec<T>), gn_vec_grid<T>(n, C), dim3(GN_THREADS), 0, s, x, dy, y, n, C, G,
+                               off, jidx, (const A*)mean, (const A*)rstd, gamma, gsums, dx);
+        else
+            hipLaunchKernelGGL((k_gn_bwd_dx<T>), dim3(grid_1d(n * C, 256)), dim3(256), 0, s, x, dy, y, n, C, G, off,
+                               jidx, (const A*)mean, (const A*)rstd, gamma, gsums, dx);
         FVDBK_LAUNCH_CHECK("k_gn_bwd_dx");
     }
     return FVDBK_OK;
