@@ -97,4 +97,18 @@ __device__ inline int32_t floordiv(int32_t a, int32_t b) {
 
 inline unsigned grid_1d(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// Compute units of the current device (cached per device); persistent kernels size their
+// grids by it.
+static inline int device_cu_count() {
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cus[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev] = n;
+    }
+    return cus[dev];
+}
+
 }  // namespace fvdbk

@@ -1096,20 +1096,6 @@ void k_conv_resident(const T* __restrict__ X, const T* __restrict__ X1, int64_t 
     }
 }
 
-// Compute units of the current device (cached per device; the persistent kernels launch
-// one workgroup per CU).
-static inline int device_cu_count() {
-    static int cus[64] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    if (cus[dev] == 0) {
-        int n = 0;
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-        cus[dev] = n;
-    }
-    return cus[dev];
-}
-
 // Launches k_conv_resident for 16-bit layers of 32 output channels, 32 or 64 input
 // channels and 27 or 8 taps in tree order; false if the layer does not fit it.  U-Net
 // level 0 (3.2M rows), MI355X, against the register-staged tile kernel: 3x3x3 32 -> 32

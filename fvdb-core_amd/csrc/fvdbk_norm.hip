@@ -13,7 +13,7 @@
 //   2. one wavefront per (grid, group) merges its blocks' partials (lanes stride over
 //      (block, channel) items, then a fixed butterfly) with Chan's pairwise update;
 //   3. y = (x - mean) * rstd * gamma + beta, optionally ReLU, one thread per element (per
-//      16-byte channel segment of GN_RPT rows in the vector form).
+//      16-byte channel segment of a row, in persistent blocks, in the vector form).
 // The backward pass reduces sum(dy) and sum(dy * xhat) per (grid, channel) the same way and
 // forms dx = rstd * (gamma dy - mean_g(gamma dy) - xhat mean_g(gamma dy xhat)).
 // Accumulation is fp32 (fp64 for fp64 data).
@@ -126,9 +126,18 @@ __global__ void k_gn_stats_final(const Welford<A>* __restrict__ part, int C, int
     const int nb = prefix[b + 1] - prefix[b];
     const int items = nb * Cg;
     Welford<A> w{(A)0, (A)0, (A)0};
-    for (int it = lane; it < items; it += 64) {
-        const int j = it / Cg, c = g * Cg + it % Cg;
-        w = wmerge(w, part[(int64_t)(prefix[b] + j) * C + c]);
+    // items lane, lane + 64, ... in order; UF loads in flight before they merge
+    constexpr int UF = 8;
+    for (int it0 = 0; it0 < items; it0 += 64 * UF) {
+        Welford<A> v[UF];
+#pragma unroll
+        for (int u = 0; u < UF; ++u) {
+            const int it = it0 + u * 64 + lane;
+            v[u] = it < items ? part[(int64_t)(prefix[b] + it / Cg) * C + g * Cg + it % Cg]
+                              : Welford<A>{(A)0, (A)0, (A)0};
+        }
+#pragma unroll
+        for (int u = 0; u < UF; ++u) w = wmerge(w, v[u]);
     }
 #pragma unroll
     for (int m = 1; m < 64; m <<= 1) {
@@ -172,56 +181,73 @@ template <typename T> union GnSeg {
     T e[16 / sizeof(T)];
 };
 
-constexpr int GN_RPT = 8;  // rows per thread in the element-wise kernels
+constexpr int GN_RPT = 4;             // rows per thread and chunk in the element-wise kernels
+constexpr int GN_STATS_THREADS = 128;  // threads per block of the vector statistics kernels
 
-// Pass 1 (vector form), same partition and output as k_gn_stats_partial: thread (q, s) runs a
-// Welford update over rows r0 + q, r0 + q + R, ... of segment s, one count and one reciprocal
-// per row shared by the E channels; the R partials of a segment then merge pairwise in LDS
-// (q with q + h for h = 1, 2, 4, ...).  The order depends only on the block's own rows.
-template <typename T>
-__global__ __launch_bounds__(GN_THREADS) void k_gn_stats_partial_vec(const T* __restrict__ x, int C,
+// Pass 1 (vector form), same partition and output as k_gn_stats_partial, one workgroup of NT
+// threads per block (two wavefronts: the blocks of a large batch then all fit on the device at
+// once): thread (q, s) sums d = x - K and d^2 over rows r0 + q, r0 + q + R, ... of segment s,
+// K the channel's value in the thread's first row (shifted data: sum d^2 - (sum d)^2 / n stays
+// exact to rounding while |mean - K| is a few standard deviations), turns them into (count,
+// mean, M2) and the R partials of a segment then merge pairwise in LDS (q with q + h for
+// h = 1, 2, 4, ...).  The order depends only on the block's own rows.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_gn_stats_partial_vec(const T* __restrict__ x, int C,
                                                                      const int64_t* __restrict__ off,
                                                                      const int32_t* __restrict__ prefix, int B,
                                                                      Welford<typename NormAcc<T>::type>* __restrict__ part) {
     using A = typename NormAcc<T>::type;
     constexpr int E = 16 / sizeof(T);
-    __shared__ A s_mean[GN_THREADS * E], s_m2[GN_THREADS * E], s_n[GN_THREADS];
+    __shared__ A s_mean[NT * E], s_m2[NT * E], s_n[NT];
     const int blk = blockIdx.x, tid = threadIdx.x;
     int b;
     int64_t r0, r1;
     gn_block_rows(off, prefix, B, blk, b, r0, r1);
     const int S = C / E, nrows = (int)(r1 - r0);
-    for (int s0 = 0; s0 < S; s0 += GN_THREADS) {
-        const int ss = min(GN_THREADS, S - s0), R = GN_THREADS / ss, cc = ss * E;
+    for (int s0 = 0; s0 < S; s0 += NT) {
+        const int ss = min(NT, S - s0), R = NT / ss, cc = ss * E;
         const int q = tid / ss, s = tid % ss;
-        A n = (A)0, mu[E], m2[E];
-#pragma unroll
-        for (int e = 0; e < E; ++e) mu[e] = m2[e] = (A)0;
         if (q < R) {
             const uint4* xs = reinterpret_cast<const uint4*>(x + r0 * C) + s0 + s;
-            auto update = [&](const GnSeg<T>& v) {
-                n += (A)1;
-                const A inv = (A)1 / n;
+            // rows q + i R, i < mine, in batches of U; the next batch loads before this one is used
+            const int mine = (nrows - q + R - 1) / R;
+            constexpr int U = 8;
+            GnSeg<T> cur[U], nxt[U];
+            auto load = [&](int i0, GnSeg<T>* v) {
 #pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    const A xv = (A)v.e[e];
-                    const A d = xv - mu[e];
-                    mu[e] += d * inv;
-                    m2[e] += d * (xv - mu[e]);
-                }
+                for (int u = 0; u < U; ++u) v[u].v = xs[(int64_t)(q + min(i0 + u, mine - 1) * R) * S];
             };
-            int r = q;
-            for (; r + 3 * R < nrows; r += 4 * R) {
-                GnSeg<T> v[4];
+            A K[E], sd[E], sq[E];
+            if (mine > 0) load(0, cur);
+            else cur[0].v = uint4{0, 0, 0, 0};
 #pragma unroll
-                for (int u = 0; u < 4; ++u) v[u].v = xs[(int64_t)(r + u * R) * S];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) update(v[u]);
+            for (int e = 0; e < E; ++e) {
+                K[e] = (A)cur[0].e[e];
+                sd[e] = sq[e] = (A)0;
             }
-            for (; r < nrows; r += R) {
-                GnSeg<T> v;
-                v.v = xs[(int64_t)r * S];
-                update(v);
+            for (int i0 = 0; i0 < mine; i0 += U) {
+                if (i0 + U < mine) load(i0 + U, nxt);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (i0 + u < mine) {
+#pragma unroll
+                        for (int e = 0; e < E; ++e) {
+                            const A d = (A)cur[u].e[e] - K[e];
+                            sd[e] += d;
+                            sq[e] += d * d;
+                        }
+                    }
+#pragma unroll
+                for (int u = 0; u < U; ++u) cur[u] = nxt[u];
+            }
+            const A n = (A)max(mine, 0);
+            const A inv = mine > 0 ? (A)1 / n : (A)0;
+            A mu[E], m2[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const A md = sd[e] * inv;
+                mu[e] = mine > 0 ? K[e] + md : (A)0;
+                m2[e] = mine > 0 ? max(sq[e] - sd[e] * md, (A)0) : (A)0;
             }
 #pragma unroll
             for (int e = 0; e < E; ++e) {
@@ -233,7 +259,7 @@ __global__ __launch_bounds__(GN_THREADS) void k_gn_stats_partial_vec(const T* __
         for (int h = 1; h < R; h <<= 1) {
             __syncthreads();
             const int pairs = (R + 2 * h - 1) / (2 * h);
-            for (int it = tid; it < pairs * cc; it += GN_THREADS) {
+            for (int it = tid; it < pairs * cc; it += NT) {
                 const int qa = (it / cc) * 2 * h, qb = qa + h, c = it % cc;
                 if (qb >= R) continue;
                 const A na = s_n[qa], nb = s_n[qb];
@@ -252,16 +278,18 @@ __global__ __launch_bounds__(GN_THREADS) void k_gn_stats_partial_vec(const T* __
             if (tid < pairs && tid * 2 * h + h < R) s_n[tid * 2 * h] += s_n[tid * 2 * h + h];
         }
         __syncthreads();
-        for (int c = tid; c < cc; c += GN_THREADS)
+        for (int c = tid; c < cc; c += NT)
             part[(int64_t)blk * C + s0 * E + c] = Welford<A>{s_n[0], s_mean[c], s_m2[c]};
         __syncthreads();
     }
 }
 
-// Rows of thread (q, s) in the element-wise vector kernels: rb + q + k R for k < GN_RPT, with
-// rb = blockIdx.x * R * GN_RPT; segment s0 + s with s0 = blockIdx.y * GN_THREADS.  The grid of
-// the first row comes from jidx (all rows are grid 0 without it); later rows advance it along
-// the offsets, since grids are contiguous and in order.
+// The element-wise vector kernels are persistent: block j handles the row chunks j,
+// j + gridDim.x, ... of CH = R * GN_RPT rows (thread (q, s): rows chunk * CH + q + k R for
+// k < GN_RPT, segment s0 + s with s0 = blockIdx.y * GN_THREADS) and loads the next chunk
+// before it writes the current one.  A thread's rows only increase, so its grid index walks
+// forward along the offsets from the grid of its first row (jidx; all rows are grid 0 without
+// it), and the statistics of a grid are loaded once per thread and grid.
 
 // Pass 3 (vector form): y = (x - mean) * rstd * gamma + beta, the same arithmetic as k_gn_apply.
 template <typename T>
@@ -273,12 +301,13 @@ __global__ __launch_bounds__(GN_THREADS) void k_gn_apply_vec(const T* __restrict
                                                              const T* __restrict__ gamma, const T* __restrict__ beta,
                                                              int relu, T* __restrict__ y) {
     using A = typename NormAcc<T>::type;
-    constexpr int E = 16 / sizeof(T);
+    constexpr int E = 16 / sizeof(T), P = GN_RPT;
     const int S = C / E, s0 = blockIdx.y * GN_THREADS;
     const int ss = min(GN_THREADS, S - s0), R = GN_THREADS / ss;
     const int q = threadIdx.x / ss, s = s0 + threadIdx.x % ss;
-    const int64_t rq = (int64_t)blockIdx.x * R * GN_RPT + q;
-    if (q >= R || rq >= n) return;
+    const int64_t CH = (int64_t)R * P, nch = (n + CH - 1) / CH;
+    int64_t ch = blockIdx.x;
+    if (q >= R || ch >= nch) return;
     const int c0 = s * E, Cg = C / G;
     A w[E], bt[E];
     int grp[E];
@@ -290,48 +319,61 @@ __global__ __launch_bounds__(GN_THREADS) void k_gn_apply_vec(const T* __restrict
     }
     const uint4* xs = reinterpret_cast<const uint4*>(x) + s;
     uint4* ys = reinterpret_cast<uint4*>(y) + s;
-    GnSeg<T> v[GN_RPT];
+    auto row = [&](int64_t c, int k) { return c * CH + q + (int64_t)k * R; };
+    GnSeg<T> cur[P];
 #pragma unroll
-    for (int k = 0; k < GN_RPT; ++k)
-        if (rq + k * R < n) v[k].v = xs[(rq + k * R) * S];
-    int b = jidx ? jidx[rq] : 0;
+    for (int k = 0; k < P; ++k) cur[k].v = xs[min(row(ch, k), n - 1) * S];
+    int b = jidx ? jidx[min(row(ch, 0), n - 1)] : 0;
     int64_t bend = off[b + 1];
     A mu[E], rs[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        mu[e] = mean[b * G + grp[e]];
-        rs[e] = rstd[b * G + grp[e]];
-    }
-#pragma unroll
-    for (int k = 0; k < GN_RPT; ++k) {
-        const int64_t r = rq + k * R;
-        if (r >= n) break;
-        if (r >= bend) {
-            do bend = off[++b + 1];
-            while (r >= bend);
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                mu[e] = mean[b * G + grp[e]];
-                rs[e] = rstd[b * G + grp[e]];
-            }
-        }
-        GnSeg<T> o;
+    auto load_grid = [&]() {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            A t = ((A)v[k].e[e] - mu[e]) * rs[e];
-            if (gamma) t = t * w[e];
-            if (beta) t = t + bt[e];
-            if (relu && !(t > (A)0) && t == t) t = (A)0;
-            o.e[e] = (T)t;
+            mu[e] = mean[b * G + grp[e]];
+            rs[e] = rstd[b * G + grp[e]];
         }
-        ys[r * S] = o.v;
+    };
+    load_grid();
+    for (;;) {
+        const int64_t nx = ch + gridDim.x;
+        GnSeg<T> nxt[P];
+        if (nx < nch) {
+#pragma unroll
+            for (int k = 0; k < P; ++k) nxt[k].v = xs[min(row(nx, k), n - 1) * S];
+        }
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const int64_t r = row(ch, k);
+            if (r >= n) break;
+            if (r >= bend) {
+                do bend = off[++b + 1];
+                while (r >= bend);
+                load_grid();
+            }
+            GnSeg<T> o;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                A t = ((A)cur[k].e[e] - mu[e]) * rs[e];
+                if (gamma) t = t * w[e];
+                if (beta) t = t + bt[e];
+                if (relu && !(t > (A)0) && t == t) t = (A)0;
+                o.e[e] = (T)t;
+            }
+            ys[r * S] = o.v;
+        }
+        if (nx >= nch) break;
+        ch = nx;
+#pragma unroll
+        for (int k = 0; k < P; ++k) cur[k] = nxt[k];
     }
 }
 
 // Backward pass 1 (vector form), same partition and output as k_gn_bwd_partial: per thread
 // sequential sums over its rows, then the R partials of a segment summed pairwise in LDS.
-template <typename T>
-__global__ __launch_bounds__(GN_THREADS) void k_gn_bwd_partial_vec(const T* __restrict__ x, const T* __restrict__ dy,
+// NT threads per workgroup as in k_gn_stats_partial_vec; HAS_Y: the forward applied a ReLU
+// and y masks dy.
+template <typename T, bool HAS_Y, int NT>
+__global__ __launch_bounds__(NT) void k_gn_bwd_partial_vec(const T* __restrict__ x, const T* __restrict__ dy,
                                                                    const T* __restrict__ y, int C, int G,
                                                                    const int64_t* __restrict__ off,
                                                                    const int32_t* __restrict__ prefix, int B,
@@ -340,14 +382,14 @@ __global__ __launch_bounds__(GN_THREADS) void k_gn_bwd_partial_vec(const T* __re
                                                                    typename NormAcc<T>::type* __restrict__ part) {
     using A = typename NormAcc<T>::type;
     constexpr int E = 16 / sizeof(T);
-    __shared__ A s_1[GN_THREADS * E], s_2[GN_THREADS * E];
+    __shared__ A s_1[NT * E], s_2[NT * E];
     const int blk = blockIdx.x, tid = threadIdx.x;
     int b;
     int64_t r0, r1;
     gn_block_rows(off, prefix, B, blk, b, r0, r1);
     const int S = C / E, nrows = (int)(r1 - r0), Cg = C / G;
-    for (int s0 = 0; s0 < S; s0 += GN_THREADS) {
-        const int ss = min(GN_THREADS, S - s0), R = GN_THREADS / ss, cc = ss * E;
+    for (int s0 = 0; s0 < S; s0 += NT) {
+        const int ss = min(NT, S - s0), R = NT / ss, cc = ss * E;
         const int q = tid / ss, s = tid % ss;
         if (q < R) {
             A a1[E], a2[E], mu[E], rs[E];
@@ -361,34 +403,41 @@ __global__ __launch_bounds__(GN_THREADS) void k_gn_bwd_partial_vec(const T* __re
             const int64_t base = r0 * S + s0 + s;
             const uint4* xs = reinterpret_cast<const uint4*>(x) + base;
             const uint4* ds = reinterpret_cast<const uint4*>(dy) + base;
-            const uint4* ps = y ? reinterpret_cast<const uint4*>(y) + base : nullptr;
+            const uint4* ps = reinterpret_cast<const uint4*>(y) + base;
             auto update = [&](const GnSeg<T>& xv, const GnSeg<T>& dv, const GnSeg<T>& yv) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
                     A d = (A)dv.e[e];
-                    if (ps && !((A)yv.e[e] > (A)0)) d = (A)0;
+                    if constexpr (HAS_Y)
+                        if (!((A)yv.e[e] > (A)0)) d = (A)0;
                     a1[e] += d;
                     a2[e] += d * (((A)xv.e[e] - mu[e]) * rs[e]);
                 }
             };
-            int r = q;
-            for (; r + 3 * R < nrows; r += 4 * R) {
-                GnSeg<T> xv[4], dv[4], yv[4];
+            const int mine = (nrows - q + R - 1) / R;
+            constexpr int U = 4, NY = HAS_Y ? U : 1;
+            GnSeg<T> cx[U], cd[U], cy[NY], nx_[U], nd[U], ny[NY];
+            auto load = [&](int i0, GnSeg<T>* xv, GnSeg<T>* dv, GnSeg<T>* yv) {
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    xv[u].v = xs[(int64_t)(r + u * R) * S];
-                    dv[u].v = ds[(int64_t)(r + u * R) * S];
-                    yv[u].v = ps ? ps[(int64_t)(r + u * R) * S] : uint4{0, 0, 0, 0};
+                for (int u = 0; u < U; ++u) {
+                    const int64_t o = (int64_t)(q + min(i0 + u, mine - 1) * R) * S;
+                    xv[u].v = xs[o];
+                    dv[u].v = ds[o];
+                    if constexpr (HAS_Y) yv[u].v = ps[o];
                 }
+            };
+            if (mine > 0) load(0, cx, cd, cy);
+            for (int i0 = 0; i0 < mine; i0 += U) {
+                if (i0 + U < mine) load(i0 + U, nx_, nd, ny);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) update(xv[u], dv[u], yv[u]);
-            }
-            for (; r < nrows; r += R) {
-                GnSeg<T> xv, dv, yv;
-                xv.v = xs[(int64_t)r * S];
-                dv.v = ds[(int64_t)r * S];
-                yv.v = ps ? ps[(int64_t)r * S] : uint4{0, 0, 0, 0};
-                update(xv, dv, yv);
+                for (int u = 0; u < U; ++u)
+                    if (i0 + u < mine) update(cx[u], cd[u], cy[HAS_Y ? u : 0]);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    cx[u] = nx_[u];
+                    cd[u] = nd[u];
+                    if constexpr (HAS_Y) cy[u] = ny[u];
+                }
             }
 #pragma unroll
             for (int e = 0; e < E; ++e) {
@@ -399,7 +448,7 @@ __global__ __launch_bounds__(GN_THREADS) void k_gn_bwd_partial_vec(const T* __re
         for (int h = 1; h < R; h <<= 1) {
             __syncthreads();
             const int pairs = (R + 2 * h - 1) / (2 * h);
-            for (int it = tid; it < pairs * cc; it += GN_THREADS) {
+            for (int it = tid; it < pairs * cc; it += NT) {
                 const int qa = (it / cc) * 2 * h, qb = qa + h, c = it % cc;
                 if (qb >= R) continue;
                 s_1[qa * cc + c] += s_1[qb * cc + c];
@@ -407,7 +456,7 @@ __global__ __launch_bounds__(GN_THREADS) void k_gn_bwd_partial_vec(const T* __re
             }
         }
         __syncthreads();
-        for (int c = tid; c < cc; c += GN_THREADS) {
+        for (int c = tid; c < cc; c += NT) {
             part[((int64_t)blk * C + s0 * E + c) * 2] = s_1[c];
             part[((int64_t)blk * C + s0 * E + c) * 2 + 1] = s_2[c];
         }
@@ -542,8 +591,9 @@ __global__ void k_gn_bwd_dx(const T* __restrict__ x, const T* __restrict__ dy, c
     dx[idx] = (T)(rstd[s] * (w * d - gsums[2 * s] / M - xh * gsums[2 * s + 1] / M));
 }
 
-// Backward pass 4 (vector form), the same arithmetic as k_gn_bwd_dx; rows as in k_gn_apply_vec.
-template <typename T>
+// Backward pass 4 (vector form), the same arithmetic as k_gn_bwd_dx, persistent as
+// k_gn_apply_vec; HAS_Y: the forward applied a ReLU and y masks dy.
+template <typename T, bool HAS_Y>
 __global__ __launch_bounds__(GN_THREADS) void k_gn_bwd_dx_vec(const T* __restrict__ x, const T* __restrict__ dy,
                                                               const T* __restrict__ y, int64_t n, int C, int G,
                                                               const int64_t* __restrict__ off,
@@ -554,12 +604,13 @@ __global__ __launch_bounds__(GN_THREADS) void k_gn_bwd_dx_vec(const T* __restric
                                                               const typename NormAcc<T>::type* __restrict__ gsums,
                                                               T* __restrict__ dx) {
     using A = typename NormAcc<T>::type;
-    constexpr int E = 16 / sizeof(T);
+    constexpr int E = 16 / sizeof(T), P = GN_RPT, NY = HAS_Y ? P : 1;
     const int S = C / E, s0 = blockIdx.y * GN_THREADS;
     const int ss = min(GN_THREADS, S - s0), R = GN_THREADS / ss;
     const int q = threadIdx.x / ss, s = s0 + threadIdx.x % ss;
-    const int64_t rq = (int64_t)blockIdx.x * R * GN_RPT + q;
-    if (q >= R || rq >= n) return;
+    const int64_t CH = (int64_t)R * P, nch = (n + CH - 1) / CH;
+    int64_t ch = blockIdx.x;
+    if (q >= R || ch >= nch) return;
     const int c0 = s * E, Cg = C / G;
     A w[E];
     int grp[E];
@@ -570,16 +621,21 @@ __global__ __launch_bounds__(GN_THREADS) void k_gn_bwd_dx_vec(const T* __restric
     }
     const uint4* xs = reinterpret_cast<const uint4*>(x) + s;
     const uint4* ds = reinterpret_cast<const uint4*>(dy) + s;
-    const uint4* ps = y ? reinterpret_cast<const uint4*>(y) + s : nullptr;
+    const uint4* ps = reinterpret_cast<const uint4*>(y) + s;
     uint4* os = reinterpret_cast<uint4*>(dx) + s;
-    GnSeg<T> xv[GN_RPT], dv[GN_RPT];
+    auto row = [&](int64_t c, int k) { return c * CH + q + (int64_t)k * R; };
+    GnSeg<T> cx[P], cd[P], cy[NY];
+    auto load_rows = [&](int64_t c, GnSeg<T>* xv, GnSeg<T>* dv, GnSeg<T>* yv) {
 #pragma unroll
-    for (int k = 0; k < GN_RPT; ++k)
-        if (rq + k * R < n) {
-            xv[k].v = xs[(rq + k * R) * S];
-            dv[k].v = ds[(rq + k * R) * S];
+        for (int k = 0; k < P; ++k) {
+            const int64_t r = min(row(c, k), n - 1) * S;
+            xv[k].v = xs[r];
+            dv[k].v = ds[r];
+            if constexpr (HAS_Y) yv[k].v = ps[r];
         }
-    int b = jidx ? jidx[rq] : 0;
+    };
+    load_rows(ch, cx, cd, cy);
+    int b = jidx ? jidx[min(row(ch, 0), n - 1)] : 0;
     int64_t bend = off[b + 1];
     A mu[E], rs[E], t1[E], t2[E];
     auto load_grid = [&]() {
@@ -594,25 +650,38 @@ __global__ __launch_bounds__(GN_THREADS) void k_gn_bwd_dx_vec(const T* __restric
         }
     };
     load_grid();
+    for (;;) {
+        const int64_t nx = ch + gridDim.x;
+        GnSeg<T> nxx[P], nxd[P], nxy[NY];
+        if (nx < nch) load_rows(nx, nxx, nxd, nxy);
 #pragma unroll
-    for (int k = 0; k < GN_RPT; ++k) {
-        const int64_t r = rq + k * R;
-        if (r >= n) break;
-        if (r >= bend) {
-            do bend = off[++b + 1];
-            while (r >= bend);
-            load_grid();
-        }
-        GnSeg<T> yv, o;
-        if (ps) yv.v = ps[r * S];
+        for (int k = 0; k < P; ++k) {
+            const int64_t r = row(ch, k);
+            if (r >= n) break;
+            if (r >= bend) {
+                do bend = off[++b + 1];
+                while (r >= bend);
+                load_grid();
+            }
+            GnSeg<T> o;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            A d = (A)dv[k].e[e];
-            if (ps && !((A)yv.e[e] > (A)0)) d = (A)0;
-            const A xh = ((A)xv[k].e[e] - mu[e]) * rs[e];
-            o.e[e] = (T)(rs[e] * (w[e] * d - t1[e] - xh * t2[e]));
+            for (int e = 0; e < E; ++e) {
+                A d = (A)cd[k].e[e];
+                if constexpr (HAS_Y)
+                    if (!((A)cy[k].e[e] > (A)0)) d = (A)0;
+                const A xh = ((A)cx[k].e[e] - mu[e]) * rs[e];
+                o.e[e] = (T)(rs[e] * (w[e] * d - t1[e] - xh * t2[e]));
+            }
+            os[r * S] = o.v;
         }
-        os[r * S] = o.v;
+        if (nx >= nch) break;
+        ch = nx;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            cx[k] = nxx[k];
+            cd[k] = nxd[k];
+            if constexpr (HAS_Y) cy[k] = nxy[k];
+        }
     }
 }
 
@@ -624,14 +693,26 @@ static bool gn_vec_ok(int C, const void* a, const void* b, const void* c, const 
     return C % (16 / (int)sizeof(T)) == 0 && (m & 15) == 0;
 }
 
-// Launch grid of the element-wise vector kernels: x over blocks of R * GN_RPT rows, y over
-// chunks of GN_THREADS segments (R of the first, widest chunk, so every chunk covers all rows).
-template <typename T>
-static dim3 gn_vec_grid(int64_t n, int C) {
+// Launch grid of the persistent element-wise vector kernels: y over chunks of GN_THREADS
+// segments; x the row chunks of R * GN_RPT rows (R of the first, widest segment chunk, so every
+// segment chunk covers all rows), at most the blocks resident on the device at once.
+template <typename T, typename K>
+static dim3 gn_vec_grid(K kernel, int64_t n, int C) {
+    static int resident[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (resident[dev] == 0) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, GN_THREADS, 0) != hipSuccess || per_cu <= 0)
+            per_cu = 1;
+        resident[dev] = per_cu * device_cu_count();
+    }
     const int S = C / (16 / (int)sizeof(T));
+    const int gy = (S + GN_THREADS - 1) / GN_THREADS;
     const int R = GN_THREADS / (S < GN_THREADS ? S : GN_THREADS);
-    return dim3((unsigned)((n + (int64_t)R * GN_RPT - 1) / ((int64_t)R * GN_RPT)),
-                (unsigned)((S + GN_THREADS - 1) / GN_THREADS));
+    const int64_t nch = (n + (int64_t)R * GN_RPT - 1) / ((int64_t)R * GN_RPT);
+    const int64_t cap = resident[dev] / gy > 0 ? resident[dev] / gy : 1;
+    return dim3((unsigned)(nch < cap ? nch : cap), (unsigned)gy);
 }
 
 template <typename T>
@@ -646,8 +727,8 @@ static int gn_forward(const T* x, int64_t n, int C, int G, const int64_t* off, c
     const bool vec = gn_vec_ok<T>(C, x, y, y, y);
     if (nb > 0) {
         if (vec)
-            hipLaunchKernelGGL((k_gn_stats_partial_vec<T>), dim3((unsigned)nb), dim3(GN_THREADS), 0, s, x, C, off,
-                               prefix, B, part);
+            hipLaunchKernelGGL((k_gn_stats_partial_vec<T, GN_STATS_THREADS>), dim3((unsigned)nb),
+                               dim3(GN_STATS_THREADS), 0, s, x, C, off, prefix, B, part);
         else
             hipLaunchKernelGGL((k_gn_stats_partial<T>), dim3((unsigned)nb), dim3(GN_THREADS), 0, s, x, C, off, prefix,
                                B, part);
@@ -659,8 +740,8 @@ static int gn_forward(const T* x, int64_t n, int C, int G, const int64_t* off, c
     FVDBK_LAUNCH_CHECK("k_gn_stats_final");
     if (n > 0) {
         if (vec)
-            hipLaunchKernelGGL((k_gn_apply_vec<T>), gn_vec_grid<T>(n, C), dim3(GN_THREADS), 0, s, x, n, C, G, off,
-                               jidx, (const A*)mean, (const A*)rstd, gamma, beta, relu, y);
+            hipLaunchKernelGGL((k_gn_apply_vec<T>), gn_vec_grid<T>(k_gn_apply_vec<T>, n, C), dim3(GN_THREADS), 0, s,
+                               x, n, C, G, off, jidx, (const A*)mean, (const A*)rstd, gamma, beta, relu, y);
         else
             hipLaunchKernelGGL((k_gn_apply<T>), dim3(grid_1d(n * C, 256)), dim3(256), 0, s, x, n, C, G, jidx,
                                (const A*)mean, (const A*)rstd, gamma, beta, relu, y);
@@ -686,9 +767,14 @@ static int gn_backward(const T* x, const T* dy, const T* y, int64_t n, int C, in
     FVDBK_LAUNCH_CHECK("k_gn_block_prefix");
     const bool vec = gn_vec_ok<T>(C, x, dy, y ? y : x, dx);
     if (nb > 0) {
-        if (vec)
-            hipLaunchKernelGGL((k_gn_bwd_partial_vec<T>), dim3((unsigned)nb), dim3(GN_THREADS), 0, s, x, dy, y, C, G,
-                               off, prefix, B, (const A*)mean, (const A*)rstd, part);
+        if (vec && y)
+            hipLaunchKernelGGL((k_gn_bwd_partial_vec<T, true, GN_STATS_THREADS>), dim3((unsigned)nb),
+                               dim3(GN_STATS_THREADS), 0, s, x, dy, y, C, G, off, prefix, B, (const A*)mean,
+                               (const A*)rstd, part);
+        else if (vec)
+            hipLaunchKernelGGL((k_gn_bwd_partial_vec<T, false, GN_STATS_THREADS>), dim3((unsigned)nb),
+                               dim3(GN_STATS_THREADS), 0, s, x, dy, y, C, G, off, prefix, B, (const A*)mean,
+                               (const A*)rstd, part);
         else
             hipLaunchKernelGGL((k_gn_bwd_partial<T>), dim3((unsigned)nb), dim3(GN_THREADS), 0, s, x, dy, y, C, G, off,
                                prefix, B, (const A*)mean, (const A*)rstd, part);
@@ -703,9 +789,14 @@ static int gn_backward(const T* x, const T* dy, const T* y, int64_t n, int C, in
                        dbl ? (double*)dgamma : nullptr, dbl ? (double*)dbeta : nullptr);
     FVDBK_LAUNCH_CHECK("k_gn_bwd_group_sums");
     if (n > 0) {
-        if (vec)
-            hipLaunchKernelGGL((k_gn_bwd_dx_vec<T>), gn_vec_grid<T>(n, C), dim3(GN_THREADS), 0, s, x, dy, y, n, C, G,
-                               off, jidx, (const A*)mean, (const A*)rstd, gamma, gsums, dx);
+        if (vec && y)
+            hipLaunchKernelGGL((k_gn_bwd_dx_vec<T, true>), gn_vec_grid<T>(k_gn_bwd_dx_vec<T, true>, n, C),
+                               dim3(GN_THREADS), 0, s, x, dy, y, n, C, G, off, jidx, (const A*)mean, (const A*)rstd,
+                               gamma, gsums, dx);
+        else if (vec)
+            hipLaunchKernelGGL((k_gn_bwd_dx_vec<T, false>), gn_vec_grid<T>(k_gn_bwd_dx_vec<T, false>, n, C),
+                               dim3(GN_THREADS), 0, s, x, dy, y, n, C, G, off, jidx, (const A*)mean, (const A*)rstd,
+                               gamma, gsums, dx);
         else
             hipLaunchKernelGGL((k_gn_bwd_dx<T>), dim3(grid_1d(n * C, 256)), dim3(256), 0, s, x, dy, y, n, C, G, off,
                                jidx, (const A*)mean, (const A*)rstd, gamma, gsums, dx);
