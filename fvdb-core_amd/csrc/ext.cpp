@@ -227,6 +227,27 @@ at::Tensor ijk_to_index(at::Tensor g_ijk, at::Tensor g_keys, at::Tensor g_masks,
     return out;
 }
 
+// int64 indices; within each query's own grid when offsets (device int64 [B+1]) is not empty.
+at::Tensor ijk_to_index64(at::Tensor g_ijk, at::Tensor g_keys, at::Tensor g_masks, at::Tensor g_wbase,
+                          at::Tensor g_hash, int64_t n_grids, at::Tensor ijk, at::Tensor jidx, at::Tensor offsets) {
+    require_hip(ijk, "ijk");
+    TORCH_CHECK_VALUE(ijk.dim() == 2 && ijk.size(1) == 3, "ijk must have shape [N, 3]");
+    ijk = ijk.to(at::kInt).contiguous();
+    const bool has_jidx = jidx.numel() > 0, local = offsets.numel() > 0;
+    if (has_jidx) jidx = jidx.to(at::kInt).contiguous();
+    TORCH_CHECK_VALUE(!local || (offsets.numel() == n_grids + 1 && offsets.scalar_type() == at::kLong &&
+                                 offsets.is_contiguous() && offsets.device() == ijk.device()),
+                      "offsets must be contiguous device int64 [B+1]");
+    c10::hip::HIPGuard guard(ijk.device().index());
+    fvdbk_grid g = make_grid(g_ijk, g_keys, g_masks, g_wbase, g_hash, n_grids);
+    at::Tensor out = at::empty({ijk.size(0)}, ijk.options().dtype(at::kLong));
+    check_rc(fvdbk_grid_ijk_to_index64(&g, ijk.data_ptr<int32_t>(), has_jidx ? jidx.data_ptr<int32_t>() : nullptr,
+                                       ijk.size(0), local ? offsets.data_ptr<int64_t>() : nullptr,
+                                       out.data_ptr<int64_t>(), stream_of(ijk)),
+             "ijk_to_index64");
+    return out;
+}
+
 at::Tensor points_to_ijk(at::Tensor points, at::Tensor jidx, at::Tensor transforms) {
     require_hip(points, "points");
     TORCH_CHECK_VALUE(points.dim() == 2 && points.size(1) == 3, "points must have shape [N, 3]");
@@ -367,6 +388,37 @@ at::Tensor gather_rows(at::Tensor src, at::Tensor idx, c10::optional<at::Tensor>
                                    fill.has_value() ? fill->data_ptr() : nullptr, out.data_ptr(), stream_of(src)),
                  "gather_rows");
     return out;
+}
+
+// Rows <-> a dense frame: frame is a [B, X, Y, Z, C] view (any strides) of the dense tensor,
+// rows [N, C] contiguous; write: frame <- rows of the voxels inside, else rows <- frame (zeros
+// outside).  ijk int32 [N, 3], jidx int32 [N] or empty (one grid), origins int32 [B, 3].
+void dense_rows(bool write, at::Tensor ijk, at::Tensor jidx, at::Tensor origins, at::Tensor frame, at::Tensor rows) {
+    require_hip(frame, "dense frame");
+    TORCH_CHECK_VALUE(frame.dim() == 5, "dense frame must be a [B, X, Y, Z, C] view");
+    TORCH_CHECK_VALUE(rows.dim() == 2 && rows.is_contiguous() && rows.size(1) == frame.size(4) &&
+                          rows.scalar_type() == frame.scalar_type() && rows.device() == frame.device(),
+                      "rows must be contiguous [N, C] with the frame's dtype and device");
+    TORCH_CHECK_VALUE(ijk.dim() == 2 && ijk.size(1) == 3 && ijk.size(0) == rows.size(0) &&
+                          ijk.scalar_type() == at::kInt && ijk.is_contiguous() && ijk.device() == frame.device(),
+                      "ijk must be contiguous int32 [N, 3] on the frame's device");
+    TORCH_CHECK_VALUE(origins.dim() == 2 && origins.size(0) == frame.size(0) && origins.size(1) == 3 &&
+                          origins.scalar_type() == at::kInt && origins.is_contiguous() &&
+                          origins.device() == frame.device(),
+                      "origins must be contiguous int32 [B, 3] on the frame's device");
+    const bool batched = jidx.numel() > 0;
+    TORCH_CHECK_VALUE(!batched || (jidx.dim() == 1 && jidx.size(0) == rows.size(0) && jidx.scalar_type() == at::kInt &&
+                                   jidx.is_contiguous() && jidx.device() == frame.device()),
+                      "jidx must be contiguous int32 [N] on the frame's device, or empty for one grid");
+    TORCH_CHECK_VALUE(batched || frame.size(0) == 1, "a batch of several grids needs jidx");
+    if (rows.size(0) == 0 || rows.size(1) == 0) return;
+    c10::hip::HIPGuard guard(frame.device().index());
+    const int64_t size[3] = {frame.size(1), frame.size(2), frame.size(3)};
+    const int64_t strides[5] = {frame.stride(0), frame.stride(1), frame.stride(2), frame.stride(3), frame.stride(4)};
+    check_rc(fvdbk_dense_rows(write ? 1 : 0, ijk.data_ptr<int32_t>(), batched ? jidx.data_ptr<int32_t>() : nullptr,
+                              rows.size(0), (int32_t)rows.size(1), origins.data_ptr<int32_t>(), size, strides,
+                              (int32_t)frame.element_size(), frame.data_ptr(), rows.data_ptr(), stream_of(frame)),
+             "dense_rows");
 }
 
 // (pairs [P,2] (source index, target row) ordered by (k, row), nbsizes [K]) from a map.
@@ -675,6 +727,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("build_grid", &build_grid, nogil);
     m.def("coarsen_grid", &coarsen_grid, nogil);
     m.def("ijk_to_index", &ijk_to_index, nogil);
+    m.def("ijk_to_index64", &ijk_to_index64, nogil);
     m.def("points_to_ijk", &points_to_ijk, nogil);
     m.def("conv_grid_candidates", &conv_grid_candidates, nogil);
     m.def("kernel_map", &kernel_map, nogil);
@@ -684,6 +737,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
           py::arg("need_argmax") = true);
     m.def("pool_avg", &pool_avg, nogil);
     m.def("gather_rows", &gather_rows, nogil, py::arg("src"), py::arg("idx"), py::arg("fill") = py::none());
+    m.def("dense_rows", &dense_rows, nogil, py::arg("write"), py::arg("ijk"), py::arg("jidx"), py::arg("origins"),
+          py::arg("frame"), py::arg("rows"));
     m.def("implicit_gemm", &implicit_gemm, nogil, pybind11::arg("in"), pybind11::arg("a"), pybind11::arg("map"),
           pybind11::arg("n_rows"), pybind11::arg("order") = pybind11::none(), pybind11::arg("bias") = pybind11::none(),
           pybind11::arg("relu") = false);

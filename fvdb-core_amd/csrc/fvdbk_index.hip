@@ -344,6 +344,20 @@ __global__ void k_ijk_to_index(fvdbk_grid g, const int32_t* __restrict__ ijk, co
     out[i] = voxel_index(g, b, ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]);
 }
 
+// 64-bit form: the index within the query's own grid (minus offsets[b]) when offsets is given.
+__global__ void k_ijk_to_index64(fvdbk_grid g, const int32_t* __restrict__ ijk, const int32_t* __restrict__ bidx,
+                                 int64_t n, const int64_t* __restrict__ offsets, int64_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t b = bidx ? bidx[i] : 0;
+    if (b < 0 || b >= g.n_grids || g.n_leaves == 0) {
+        out[i] = -1;
+        return;
+    }
+    const int32_t v = voxel_index(g, b, ijk[3 * i], ijk[3 * i + 1], ijk[3 * i + 2]);
+    out[i] = v < 0 ? (int64_t)-1 : (int64_t)v - (offsets ? offsets[b] : 0);
+}
+
 template <typename T>
 __global__ void k_points_to_ijk(const T* __restrict__ pts, const int32_t* __restrict__ bidx, int64_t n,
                                 const double* __restrict__ xf, int32_t* __restrict__ ijk) {
@@ -1015,6 +1029,17 @@ int fvdbk_grid_ijk_to_index(const fvdbk_grid* grid, const int32_t* ijk, const in
     hipLaunchKernelGGL(k_ijk_to_index, dim3(grid_1d(n, 256)), dim3(256), 0, stream, *grid, ijk, batch_idx, n,
                        out_index);
     FVDBK_LAUNCH_CHECK("k_ijk_to_index");
+    return FVDBK_OK;
+}
+
+int fvdbk_grid_ijk_to_index64(const fvdbk_grid* grid, const int32_t* ijk, const int32_t* batch_idx, int64_t n,
+                              const int64_t* offsets, int64_t* out_index, hipStream_t stream) {
+    FVDBK_REQUIRE(grid && n >= 0, FVDBK_EINVAL, "ijk_to_index64: bad arguments");
+    if (n == 0) return FVDBK_OK;
+    FVDBK_REQUIRE(ijk && out_index, FVDBK_EINVAL, "ijk_to_index64: null argument");
+    hipLaunchKernelGGL(k_ijk_to_index64, dim3(grid_1d(n, 256)), dim3(256), 0, stream, *grid, ijk, batch_idx, n,
+                       offsets, out_index);
+    FVDBK_LAUNCH_CHECK("k_ijk_to_index64");
     return FVDBK_OK;
 }
 

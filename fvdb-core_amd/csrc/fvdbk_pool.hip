@@ -178,6 +178,45 @@ __global__ __launch_bounds__(256) void k_gather_rows(const U* __restrict__ src, 
     out[t] = v;
 }
 
+// Rows <-> dense frame.  Voxel i of grid b = jidx[i] (0 without jidx) sits at p = ijk[i] -
+// origin[b]; inside when 0 <= p < (dx, dy, dz).  Unit c of its row (one element, or 16 bytes
+// of channels when channels are contiguous in both) <-> dense[b sb + p.x sx + p.y sy + p.z sz +
+// c sc], strides in units.  WRITE stores the rows of voxels inside into the frame; otherwise the
+// rows read the frame, zeros for voxels outside.  CFAST: consecutive threads take consecutive
+// units of a row (channels contiguous in the frame), else consecutive voxels of one unit.
+struct DenseFrame {
+    int64_t sb, sx, sy, sz, sc;
+    int64_t dx, dy, dz;
+};
+
+template <typename U, bool WRITE, bool CFAST>
+__global__ __launch_bounds__(256) void k_dense_rows(const int32_t* __restrict__ ijk, const int32_t* __restrict__ jidx,
+                                                    int64_t n, int units, const int32_t* __restrict__ origin,
+                                                    DenseFrame f, U* __restrict__ dense, U* __restrict__ rows) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * units) return;
+    int64_t i;
+    int c;
+    if (CFAST) {
+        i = t / units;
+        c = (int)(t - i * units);
+    } else {
+        c = (int)(t / n);
+        i = t - (int64_t)c * n;
+    }
+    const int b = jidx ? jidx[i] : 0;
+    const int64_t x = (int64_t)ijk[3 * i] - origin[3 * b], y = (int64_t)ijk[3 * i + 1] - origin[3 * b + 1],
+                  z = (int64_t)ijk[3 * i + 2] - origin[3 * b + 2];
+    const bool inside = x >= 0 && x < f.dx && y >= 0 && y < f.dy && z >= 0 && z < f.dz;
+    const int64_t d = b * f.sb + x * f.sx + y * f.sy + z * f.sz + c * f.sc;
+    U* row = rows + i * units + c;
+    if constexpr (WRITE) {
+        if (inside) dense[d] = *row;
+    } else {
+        *row = inside ? dense[d] : U{};
+    }
+}
+
 }  // namespace fvdbk
 
 using namespace fvdbk;
@@ -210,6 +249,51 @@ int fvdbk_gather_rows(const void* src, int64_t n_src, int64_t row_bytes, const i
     else if (row_bytes % 4 == 0 && (al & 3u) == 0) go(uint32_t{});
     else go(uint8_t{});
     FVDBK_LAUNCH_CHECK("k_gather_rows");
+    return FVDBK_OK;
+}
+
+int fvdbk_dense_rows(int write, const int32_t* ijk, const int32_t* jidx, int64_t n, int32_t channels,
+                     const int32_t* origins, const int64_t size[3], const int64_t strides[5], int32_t elem_bytes,
+                     void* dense, void* rows, hipStream_t stream) {
+    FVDBK_REQUIRE(n >= 0 && channels > 0 && size && strides, FVDBK_EINVAL, "dense rows: bad sizes");
+    FVDBK_REQUIRE(elem_bytes == 1 || elem_bytes == 2 || elem_bytes == 4 || elem_bytes == 8, FVDBK_EINVAL,
+                  "dense rows: element size %d not 1, 2, 4 or 8", elem_bytes);
+    FVDBK_REQUIRE(size[0] >= 0 && size[1] >= 0 && size[2] >= 0, FVDBK_EINVAL, "dense rows: negative frame size");
+    if (n == 0) return FVDBK_OK;
+    FVDBK_REQUIRE(ijk && origins && dense && rows, FVDBK_EINVAL, "dense rows: null argument");
+    const int64_t eb = elem_bytes;
+    const bool cfast = strides[4] == 1;
+    const bool vec = cfast && (channels * eb) % 16 == 0 && (((uintptr_t)dense | (uintptr_t)rows) & 15u) == 0 &&
+                     (strides[0] * eb) % 16 == 0 && (strides[1] * eb) % 16 == 0 && (strides[2] * eb) % 16 == 0 &&
+                     (strides[3] * eb) % 16 == 0;
+    auto go = [&](auto unit) {
+        using U = decltype(unit);
+        const int64_t per = (int64_t)sizeof(U) / eb;
+        const DenseFrame f{strides[0] / per, strides[1] / per, strides[2] / per, strides[3] / per,
+                           cfast ? 1 : strides[4], size[0], size[1], size[2]};
+        const int units = (int)(channels / per);
+        const dim3 grid(grid_1d(n * units, 256));
+        U* d = (U*)dense;
+        U* r = (U*)rows;
+        if (write && cfast)
+            hipLaunchKernelGGL((k_dense_rows<U, true, true>), grid, dim3(256), 0, stream, ijk, jidx, n, units, origins,
+                               f, d, r);
+        else if (write)
+            hipLaunchKernelGGL((k_dense_rows<U, true, false>), grid, dim3(256), 0, stream, ijk, jidx, n, units,
+                               origins, f, d, r);
+        else if (cfast)
+            hipLaunchKernelGGL((k_dense_rows<U, false, true>), grid, dim3(256), 0, stream, ijk, jidx, n, units,
+                               origins, f, d, r);
+        else
+            hipLaunchKernelGGL((k_dense_rows<U, false, false>), grid, dim3(256), 0, stream, ijk, jidx, n, units,
+                               origins, f, d, r);
+    };
+    if (vec) go(uint4{});
+    else if (eb == 1) go(uint8_t{});
+    else if (eb == 2) go(uint16_t{});
+    else if (eb == 4) go(uint32_t{});
+    else go(uint64_t{});
+    FVDBK_LAUNCH_CHECK("k_dense_rows");
     return FVDBK_OK;
 }
 

@@ -17,6 +17,7 @@ from typing import List, Optional, Sequence, Tuple, Union
 
 import torch
 
+from ._dense import read_dense, write_dense
 from ._lib import ext, require_device
 from .jagged_tensor import JaggedTensor
 from .types import resolve_device, to_vec3f_batch, to_vec3i
@@ -28,7 +29,7 @@ class _GridIndex:
     """Device tensors of one built grid batch (the fvdbk_grid of the C ABI)."""
 
     __slots__ = ("ijk", "jidx", "keys", "masks", "wbase", "hash", "counts", "joffsets", "n_grids", "voxel_sizes",
-                 "origins", "device")
+                 "origins", "device", "bbox_host")
 
     def __init__(self, ijk, jidx, joffsets_host, joffsets, keys, masks, wbase, hash_, n_grids, voxel_sizes,
                  origins):
@@ -44,6 +45,7 @@ class _GridIndex:
         self.joffsets = joffsets
         self.voxel_sizes = voxel_sizes  # float64 CPU [B,3]
         self.origins = origins
+        self.bbox_host = None  # total bounding box, host int64 [2,3], read from the device once
 
     def tensors(self) -> List[torch.Tensor]:
         return [self.ijk, self.keys, self.masks, self.wbase, self.hash]
@@ -74,6 +76,10 @@ def _as_jagged(x) -> JaggedTensor:
     if isinstance(x, (list, tuple)):
         return JaggedTensor(list(x))
     raise TypeError(f"expected a JaggedTensor or torch.Tensor, got {type(x)}")
+
+
+_EMPTY_I32 = torch.empty(0, dtype=torch.int32)
+_EMPTY_I64 = torch.empty(0, dtype=torch.int64)
 
 
 def _full_jidx(j: JaggedTensor) -> torch.Tensor:
@@ -406,11 +412,9 @@ class GridBatch:
         j = _as_jagged(ijk)
         if j.num_tensors != self.grid_count:
             raise ValueError("ijk must have one list per grid")
-        jid = _full_jidx(j).to(self.device)
-        idx = ext.ijk_to_index(*self._impl.tensors(), self.grid_count, j.jdata.to(self.device), jid).long()
-        if not cumulative:
-            base = self._impl.joffsets[jid.long()]
-            idx = torch.where(idx >= 0, idx - base, idx)
+        jid = j.jidx.to(self.device, torch.int32) if j.num_tensors > 1 else _EMPTY_I32.to(self.device)
+        off = _EMPTY_I64.to(self.device) if cumulative else self._impl.joffsets
+        idx = ext.ijk_to_index64(*self._impl.tensors(), self.grid_count, j.jdata.to(self.device), jid, off)
         return j.jagged_like(idx)
 
     def ijk_to_inv_index(self, ijk, cumulative: bool = False) -> JaggedTensor:
@@ -565,9 +569,12 @@ class GridBatch:
         chooses them: the origin is `min_coord` (one per grid or broadcast) or the minimum of
         the batch's total bounding box; the size is `grid_size` or the total bounding box's
         extent (max - min + 1), whatever `min_coord` is."""
-        bbox = self.total_bbox.to(torch.int64)
+        impl = self._impl
+        if impl.bbox_host is None and (min_coord is None or grid_size is None):
+            impl.bbox_host = self.total_bbox.to(torch.int64).cpu()
+        bbox = impl.bbox_host
         if min_coord is None:
-            lo = bbox[0].reshape(1, 3).expand(self.grid_count, 3)
+            lo = bbox[0].reshape(1, 3).expand(self.grid_count, 3).to(self.device)
         else:
             lo = self._coords_per_grid(min_coord, "min_coord")
         if grid_size is None:
@@ -579,6 +586,13 @@ class GridBatch:
     def write_to_dense_xyzc(self, sparse_data, min_coord=None, grid_size=None) -> torch.Tensor:
         """Dense [B, X, Y, Z, *] tensor of the voxels' rows (zeros elsewhere); voxel ijk of grid b
         lands at ijk - min_coord[b] when inside the frame (``autograd/ReadIntoDense.cpp:20-80``)."""
+        return self._write_dense(sparse_data, min_coord, grid_size, "xyzc")
+
+    def write_to_dense_czyx(self, sparse_data, min_coord=None, grid_size=None) -> torch.Tensor:
+        """[B, C, Z, Y, X] layout of ``write_to_dense_xyzc`` (the row's dimensions flattened into C)."""
+        return self._write_dense(sparse_data, min_coord, grid_size, "czyx")
+
+    def _write_dense(self, sparse_data, min_coord, grid_size, layout) -> torch.Tensor:
         data = sparse_data.jdata if isinstance(sparse_data, JaggedTensor) else sparse_data
         if data.dim() < 2:
             raise ValueError("sparse_data must have shape [num_voxels, *]")
@@ -587,45 +601,28 @@ class GridBatch:
         if self.total_voxels == 0:
             raise RuntimeError("Empty grid")
         lo, size = self._dense_frame(min_coord, grid_size)
-        b = self._impl.jidx.long()
-        rel = self._impl.ijk.long() - lo[b]
-        inside = ((rel >= 0) & (rel < torch.tensor(size, device=self.device))).all(1)
-        out = torch.zeros((self.grid_count,) + size + tuple(data.shape[1:]), dtype=data.dtype, device=data.device)
-        sel = inside.nonzero().reshape(-1)
-        r = rel[sel]
-        return out.index_put((b[sel], r[:, 0], r[:, 1], r[:, 2]), data[sel])
-
-    def write_to_dense_czyx(self, sparse_data, min_coord=None, grid_size=None) -> torch.Tensor:
-        """[B, C, Z, Y, X] layout of ``write_to_dense_xyzc`` (one channel dimension)."""
-        d = self.write_to_dense_xyzc(sparse_data, min_coord, grid_size)
-        d = d.reshape(d.shape[:4] + (-1,))
-        return d.permute(0, 4, 3, 2, 1)
+        return write_dense(data, self, lo, size, layout)
 
     def read_from_dense_xyzc(self, dense_data: torch.Tensor, dense_origins=0) -> JaggedTensor:
         """Rows [N, *] read from a dense [B, X, Y, Z, *] tensor at ijk - dense_origins[b]; voxels
         outside it read zeros (``autograd/ReadFromDense.cpp:18-60``)."""
         if dense_data.dim() <= 4 or dense_data.shape[0] != self.grid_count:
             raise ValueError("dense data must have shape [B, W, H, D, *]")
-        if self.total_voxels == 0:
-            raise RuntimeError("Empty grid")
-        lo = self._coords_per_grid(dense_origins, "dense_origins")
-        lo = lo.expand(self.grid_count, 3)
-        b = self._impl.jidx.long()
-        rel = self._impl.ijk.long() - lo[b]
-        size = torch.tensor(dense_data.shape[1:4], device=self.device)
-        inside = ((rel >= 0) & (rel < size)).all(1)
-        out = torch.zeros((self.total_voxels,) + tuple(dense_data.shape[4:]), dtype=dense_data.dtype,
-                          device=dense_data.device)
-        sel = inside.nonzero().reshape(-1)
-        r = rel[sel]
-        out = out.index_put((sel,), dense_data[b[sel], r[:, 0], r[:, 1], r[:, 2]])
-        return self.jagged_like(out)
+        return self._read_dense(dense_data, dense_origins, "xyzc")
 
     def read_from_dense_czyx(self, dense_data: torch.Tensor, dense_origins=0) -> JaggedTensor:
         """``read_from_dense_xyzc`` of a dense [B, C, Z, Y, X] tensor."""
-        if dense_data.dim() != 5:
+        if dense_data.dim() != 5 or dense_data.shape[0] != self.grid_count:
             raise ValueError("dense data must have shape [B, C, D, H, W]")
-        return self.read_from_dense_xyzc(dense_data.permute(0, 4, 3, 2, 1), dense_origins)
+        return self._read_dense(dense_data, dense_origins, "czyx")
+
+    def _read_dense(self, dense_data, dense_origins, layout) -> JaggedTensor:
+        if self.total_voxels == 0:
+            raise RuntimeError("Empty grid")
+        if dense_data.device != self.device:
+            raise ValueError("dense data must be on the grid's device")
+        lo = self._coords_per_grid(dense_origins, "dense_origins").expand(self.grid_count, 3)
+        return self.jagged_like(read_dense(dense_data, self, lo, layout))
 
     # ----------------------------------------------------------------------------
     # convolution entry points

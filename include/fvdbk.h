@@ -164,6 +164,11 @@ int fvdbk_grid_offsets(const uint64_t* leaf_keys, const int64_t* leaf_incl, int6
  * src/fvdb/detail/ops/IjkToIndex.cu (GridBatch.ijk_to_index, cumulative indices). */
 int fvdbk_grid_ijk_to_index(const fvdbk_grid* grid, const int32_t* ijk, const int32_t* batch_idx, int64_t n,
                             int32_t* out_index, hipStream_t stream);
+/* The same as int64 (the reference's index type), with the index taken within the query's own
+ * grid, minus offsets[batch_idx[i]], when offsets (device int64 [n_grids+1]) is not NULL
+ * (GridBatch.ijk_to_index with cumulative=False); -1 stays -1. */
+int fvdbk_grid_ijk_to_index64(const fvdbk_grid* grid, const int32_t* ijk, const int32_t* batch_idx, int64_t n,
+                              const int64_t* offsets, int64_t* out_index, hipStream_t stream);
 
 /* World points -> voxel coordinates, ijk = floor(p * scale + translate + 0.5) in the
  * point precision (src/fvdb/detail/ops/BuildGridFromPoints.cu:28-44 with
@@ -231,6 +236,19 @@ int fvdbk_pool_avg(fvdbk_dtype dtype, const void* in, int32_t channels, const in
  * or fill.  Copies in 16-byte units when rows and pointers allow, else 4-byte, else bytes. */
 int fvdbk_gather_rows(const void* src, int64_t n_src, int64_t row_bytes, const int32_t* idx, int64_t n,
                       const void* fill, void* out, hipStream_t stream);
+
+/* Sparse rows <-> dense frame (replaces the voxel callbacks of src/fvdb/detail/ops/ReadIntoDense.cu:17-91
+ * and ops/ReadFromDense.cu:15-89 behind GridBatch.write_to_dense_* / read_from_dense_*): voxel i of
+ * grid b = jidx[i] (0 when jidx is NULL) sits at p = ijk[i] - origins[b] (ijk device int32 [n][3],
+ * origins device int32 [n_grids][3]).  When 0 <= p < size (the frame's extent on x, y, z) channel c
+ * of its row pairs with the dense element strides[0] b + strides[1] p.x + strides[2] p.y +
+ * strides[3] p.z + strides[4] c (element strides: a [B, X, Y, Z, C] and a [B, C, Z, Y, X] tensor
+ * alike).  rows [n][channels] row-major; elements of elem_bytes bytes (1, 2, 4 or 8).
+ * write != 0: the frame gets the rows of the voxels inside (other elements untouched: clear it
+ * first); write == 0: the rows get the frame's values, zeros for voxels outside. */
+int fvdbk_dense_rows(int write, const int32_t* ijk, const int32_t* jidx, int64_t n, int32_t channels,
+                     const int32_t* origins, const int64_t size[3], const int64_t strides[5], int32_t elem_bytes,
+                     void* dense, void* rows, hipStream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Gather-scatter neighbour map (replaces SparseConvPackInfo::buildGatherScatter,

@@ -223,6 +223,52 @@ def test_dense_read_write_vs_oracle():
         grid.write_to_dense_xyzc(torch.zeros(N, device="cuda"))
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16, torch.float16, torch.uint8])
+@pytest.mark.parametrize("row", [(8,), (3,), (2, 4)])
+def test_dense_io_dtypes_layouts(dtype, row):
+    """Every element size, rows of 16 bytes (16-byte path) and of other widths, several row
+    dimensions, a strided (non-contiguous) dense input and the gradients of both layouts."""
+    fvdb = _fvdb()
+    g = torch.Generator().manual_seed(11)
+    raw = [torch.randint(0, 24, (900, 3), generator=g, dtype=torch.int32),
+           torch.randint(-8, 10, (400, 3), generator=g, dtype=torch.int32)]
+    grid = fvdb.GridBatch.from_ijk(fvdb.JaggedTensor([r.cuda() for r in raw]))
+    parts = grid_parts(grid)
+    N = grid.total_voxels
+    data = torch.randn((N,) + row, generator=g) * 40
+    data = (data.abs() if dtype == torch.uint8 else data).to(dtype)
+    lo = np.array([[1, -2, 0], [-9, -3, -1]])
+    size = (20, 17, 23)
+    print(f"[dense io] dtype={dtype} row={row}", flush=True)
+    d = grid.write_to_dense_xyzc(grid.jagged_like(data.cuda()), min_coord=lo.tolist(), grid_size=size)
+    flat = data.reshape(N, -1).double().numpy()
+    ref = oracle.write_to_dense(parts, flat, lo, size)
+    assert d.shape == (2,) + size + row and d.dtype == dtype
+    assert np.array_equal(d.reshape(2, *size, -1).double().cpu().numpy(), ref)
+    dc = grid.write_to_dense_czyx(grid.jagged_like(data.cuda()), min_coord=lo.tolist(), grid_size=size)
+    assert dc.is_contiguous() and torch.equal(dc, d.reshape(2, *size, -1).permute(0, 4, 3, 2, 1))
+    want = oracle.read_from_dense(parts, ref, lo)
+    back = grid.read_from_dense_xyzc(d, dense_origins=lo.tolist())
+    assert back.jdata.shape == (N,) + row
+    assert np.array_equal(back.jdata.reshape(N, -1).double().cpu().numpy(), want)
+    # a strided view: the xyzc frame seen as czyx without a copy
+    view = d.reshape(2, *size, -1).permute(0, 4, 3, 2, 1)
+    assert not view.is_contiguous()
+    back_v = grid.read_from_dense_czyx(view, dense_origins=lo.tolist())
+    assert np.array_equal(back_v.jdata.double().cpu().numpy(), want)
+    if dtype.is_floating_point:
+        w = torch.randn(dc.shape, generator=g).to(dtype).cuda()
+        x = data.cuda().requires_grad_(True)
+        (grid.write_to_dense_czyx(grid.jagged_like(x), min_coord=lo.tolist(), grid_size=size) * w).sum().backward()
+        gw = grid.read_from_dense_czyx(w, dense_origins=lo.tolist()).jdata
+        assert torch.equal(x.grad.reshape(N, -1), gw)
+        dd = d.clone().requires_grad_(True)
+        r = torch.randn((N,) + row, generator=g).to(dtype).cuda()
+        (grid.read_from_dense_xyzc(dd, dense_origins=lo.tolist()).jdata * r).sum().backward()
+        want_g = grid.write_to_dense_xyzc(grid.jagged_like(r), min_coord=lo.tolist(), grid_size=size)
+        assert torch.equal(dd.grad, want_g)
+
+
 def test_ijk_to_index_vs_oracle():
     fvdb = _fvdb()
     g = torch.Generator().manual_seed(5)
