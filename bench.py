@@ -624,6 +624,14 @@ def run_rows(torch, fvdb, dev, args):
         # forward 3 passes; backward reads x and dy twice and writes dx
         put("group_norm_32_fwd_bwd", t, 8 * N * C * b, N)
         del xg, dyg
+        # what torch's own streaming kernels reach over the same array: a copy (read + write)
+        # and a full sum (read only), the practical ceilings for the rows above
+        yb = torch.empty_like(xb)
+        t = _median_ms(torch, lambda: yb.copy_(xb), 10)
+        put("torch_copy_reference", t, 2 * N * C * b, N)
+        t = _median_ms(torch, lambda: xb.sum(dtype=torch.float32), 10)
+        put("torch_sum_reference", t, N * C * b, N)
+        del yb
         t = _median_ms(torch, lambda: grid.ijk_to_index(ijk), 10)
         put("ijk_to_index", t, N * (12 + 8), N)
         # dense I/O on one grid whose box stays small (8 channels)
