@@ -104,12 +104,15 @@ std::vector<at::Tensor> finish_grid(const at::Tensor& lkeys, int64_t L, int64_t 
     at::Tensor masks = at::empty({L, 8}, o64);
     fill_masks(reinterpret_cast<const fvdbk_hslot*>(hash.data_ptr<int64_t>()), cap,
                reinterpret_cast<uint64_t*>(masks.data_ptr<int64_t>()), L);
-    at::Tensor counts = at::empty({L}, o32);
-    check_rc(fvdbk_grid_leaf_counts(reinterpret_cast<const uint64_t*>(masks.data_ptr<int64_t>()), L,
-                                    counts.data_ptr<int32_t>(), s),
-             "grid_leaf_counts");
-    at::Tensor incl = at::cumsum(counts, 0, at::kLong);
-    at::Tensor start = (incl - counts.to(at::kLong)).contiguous();
+    at::Tensor incl = at::empty({L}, o64), start = at::empty({L}, o64);
+    if (L > 0) {
+        const size_t scan_bytes = fvdbk_grid_leaf_scan_workspace(L);
+        at::Tensor scan_ws = at::empty({(int64_t)std::max<size_t>(scan_bytes, 1)}, lkeys.options().dtype(at::kByte));
+        check_rc(fvdbk_grid_leaf_scan(reinterpret_cast<const uint64_t*>(masks.data_ptr<int64_t>()), L,
+                                      start.data_ptr<int64_t>(), incl.data_ptr<int64_t>(), scan_ws.data_ptr(),
+                                      scan_bytes, s),
+                 "grid_leaf_scan");
+    }
     // grid offsets on the device; one read gives the total and the per-grid counts
     at::Tensor jo = at::empty({n_grids + 1}, o64);
     check_rc(fvdbk_grid_offsets(reinterpret_cast<const uint64_t*>(ukeys.data_ptr<int64_t>()), incl.data_ptr<int64_t>(),

@@ -11,6 +11,9 @@
 
 #include <hipcub/block/block_radix_sort.hpp>
 #include <hipcub/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 
 #include "fvdbk_common.h"
 
@@ -246,6 +249,24 @@ __global__ void k_leaf_counts(const uint64_t* __restrict__ masks, int64_t n_leav
 #pragma unroll
     for (int w = 0; w < 8; ++w) c += __popcll(masks[l * 8 + w]);
     counts[l] = c;
+}
+
+// Active voxels of leaf l (the popcount of its 8 mask words), as a scan input.
+struct LeafPopcount {
+    const uint64_t* masks;
+    __host__ __device__ int64_t operator()(int64_t l) const {
+        int64_t c = 0;
+#pragma unroll
+        for (int w = 0; w < 8; ++w) c += __popcll(masks[l * 8 + w]);
+        return c;
+    }
+};
+
+// leaf_start[l] = leaf_incl[l] - (voxels of leaf l): the exclusive scan from the inclusive one.
+__global__ void k_leaf_start(const uint64_t* __restrict__ masks, const int64_t* __restrict__ incl, int64_t n_leaves,
+                             int64_t* __restrict__ start) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n_leaves) start[l] = incl[l] - LeafPopcount{masks}(l);
 }
 
 __global__ void k_zero_i32(int32_t* __restrict__ p, int64_t n) {
@@ -1003,6 +1024,39 @@ int fvdbk_grid_leaf_counts(const uint64_t* leaf_masks, int64_t n_leaves, int32_t
     hipLaunchKernelGGL(k_leaf_counts, dim3(grid_1d(n_leaves, 256)), dim3(256), 0, stream, leaf_masks, n_leaves,
                        counts);
     FVDBK_LAUNCH_CHECK("k_leaf_counts");
+    return FVDBK_OK;
+}
+
+using LeafCountIter = rocprim::transform_iterator<rocprim::counting_iterator<int64_t>, LeafPopcount, int64_t>;
+
+static LeafCountIter leaf_count_iter(const uint64_t* masks) {
+    return LeafCountIter(rocprim::counting_iterator<int64_t>(0), LeafPopcount{masks});
+}
+
+size_t fvdbk_grid_leaf_scan_workspace(int64_t n_leaves) {
+    if (n_leaves <= 0 || n_leaves > INT32_MAX) return 0;
+    size_t bytes = 0;
+    if (rocprim::inclusive_scan(nullptr, bytes, leaf_count_iter(nullptr), static_cast<int64_t*>(nullptr),
+                                (size_t)n_leaves, rocprim::plus<int64_t>(), (hipStream_t)0) != hipSuccess)
+        return 0;
+    return bytes;
+}
+
+int fvdbk_grid_leaf_scan(const uint64_t* leaf_masks, int64_t n_leaves, int64_t* leaf_start, int64_t* leaf_incl,
+                         void* workspace, size_t workspace_bytes, hipStream_t stream) {
+    FVDBK_REQUIRE(n_leaves >= 0 && n_leaves <= INT32_MAX, FVDBK_EINVAL, "leaf count %lld out of range",
+                  (long long)n_leaves);
+    if (n_leaves == 0) return FVDBK_OK;
+    FVDBK_REQUIRE(leaf_masks && leaf_start && leaf_incl, FVDBK_EINVAL, "leaf scan: null argument");
+    size_t need = fvdbk_grid_leaf_scan_workspace(n_leaves);
+    FVDBK_REQUIRE(need > 0 && workspace != nullptr && workspace_bytes >= need, FVDBK_EINVAL,
+                  "leaf scan workspace of %zu bytes is smaller than the %zu needed", workspace_bytes, need);
+    const hipError_t e = rocprim::inclusive_scan(workspace, need, leaf_count_iter(leaf_masks), leaf_incl,
+                                                 (size_t)n_leaves, rocprim::plus<int64_t>(), stream);
+    FVDBK_REQUIRE(e == hipSuccess, FVDBK_EHIP, "leaf scan: %s", hipGetErrorString(e));
+    hipLaunchKernelGGL(k_leaf_start, dim3(grid_1d(n_leaves, 256)), dim3(256), 0, stream, leaf_masks, leaf_incl,
+                       n_leaves, leaf_start);
+    FVDBK_LAUNCH_CHECK("k_leaf_start");
     return FVDBK_OK;
 }
 

@@ -90,6 +90,7 @@ const char* fvdbk_version(void);
  *   4. fvdbk_grid_leaf_masks    512-bit masks (OR of the input voxels)
  *   5. fvdbk_grid_leaf_counts   active voxels per leaf
  *   6. caller exclusive-scans the counts -> leaf_start[n_leaves]
+ *      (or 5-6 in one call: fvdbk_grid_leaf_scan, inclusive and exclusive scans)
  *   7. fvdbk_grid_finalize      word bases, ijk and grid index of every voxel
  * ------------------------------------------------------------------------- */
 
@@ -146,6 +147,13 @@ int fvdbk_grid_coarsen_masks(const uint64_t* fine_keys, const uint64_t* fine_mas
                              int64_t n_leaves, hipStream_t stream);
 
 int fvdbk_grid_leaf_counts(const uint64_t* leaf_masks, int64_t n_leaves, int32_t* counts, hipStream_t stream);
+
+/* Steps 5-6 in one call: leaf_incl[l] = voxels in leaves 0..l and leaf_start[l] = voxels in
+ * leaves 0..l-1 (device int64 [n_leaves] each), a device scan over the leaves' mask popcounts.
+ * The workspace (device scratch) is fvdbk_grid_leaf_scan_workspace(n_leaves) bytes. */
+size_t fvdbk_grid_leaf_scan_workspace(int64_t n_leaves);
+int fvdbk_grid_leaf_scan(const uint64_t* leaf_masks, int64_t n_leaves, int64_t* leaf_start, int64_t* leaf_incl,
+                         void* workspace, size_t workspace_bytes, hipStream_t stream);
 
 /* Writes leaf_word_base [n_leaves][8], ijk_out [n_voxels][3], jidx_out [n_voxels]
  * (grid index per voxel; may be NULL) and grid_counts [n_grids] (voxels per grid; may
