@@ -37,12 +37,15 @@ CASES = [((2, 2, 2), (2, 2, 2)), ((3, 3, 3), (2, 2, 2)), ((2, 2, 2), (1, 1, 1)),
 
 @pytest.mark.parametrize("f,s", CASES)
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.float16, torch.bfloat16])
-def test_pool_vs_oracle(f, s, dtype):
+@pytest.mark.parametrize("C", [5, 16])
+def test_pool_vs_oracle(f, s, dtype, C):
+    """C = 5: the per-element kernels; C = 16: rows of whole 16-byte segments in every dtype
+    (the vector kernel)."""
     fvdb = _fvdb()
-    rng = np.random.default_rng(sum(f) + 10 * sum(s))
-    print(f"[case] pool f={f} s={s} {dtype}", flush=True)
+    rng = np.random.default_rng(sum(f) + 10 * sum(s) + C)
+    print(f"[case] pool f={f} s={s} {dtype} C={C}", flush=True)
     fine, fine_np = grids(fvdb, rng, [700, 300], 40)
-    x = torch.from_numpy(rng.standard_normal((fine.total_voxels, 5))).to("cuda", dtype)
+    x = torch.from_numpy(rng.standard_normal((fine.total_voxels, C))).to("cuda", dtype)
     out_max, coarse = fine.max_pool(f, fine.jagged_like(x), stride=s)
     out_avg, coarse2 = fine.avg_pool(f, fine.jagged_like(x), stride=s, coarse_grid=coarse)
     offs = coarse.joffsets.cpu().tolist()
@@ -59,11 +62,11 @@ def test_pool_vs_oracle(f, s, dtype):
     assert torch.equal(out_avg.jdata.cpu(), ref_avg)
     # backward by definition
     xg = x.clone().requires_grad_(True)
-    gy = torch.from_numpy(rng.standard_normal((coarse.total_voxels, 5))).to("cuda", dtype)
+    gy = torch.from_numpy(rng.standard_normal((coarse.total_voxels, C))).to("cuda", dtype)
     fine.max_pool(f, fine.jagged_like(xg), stride=s, coarse_grid=coarse)[0].jdata.backward(gy)
-    want = torch.zeros((fine.total_voxels, 5), dtype=torch.float64)
+    want = torch.zeros((fine.total_voxels, C), dtype=torch.float64)
     ok = ref_arg >= 0
-    want.view(-1).index_add_(0, (ref_arg * 5 + torch.arange(5))[ok], gy.cpu().double()[ok])
+    want.view(-1).index_add_(0, (ref_arg * C + torch.arange(C))[ok], gy.cpu().double()[ok])
     # windows overlap when a stride is below the pool factor: several contributions then
     # meet in one voxel and are summed by atomics in an unspecified order
     overlap = any(a < b for a, b in zip(s, f))
@@ -72,7 +75,7 @@ def test_pool_vs_oracle(f, s, dtype):
     xg2 = x.clone().requires_grad_(True)
     fine.avg_pool(f, fine.jagged_like(xg2), stride=s, coarse_grid=coarse)[0].jdata.backward(gy)
     gs = (gy.cpu() * torch.tensor(1.0 / (f[0] * f[1] * f[2]), dtype=dtype)).double()
-    want2 = torch.zeros((fine.total_voxels, 5), dtype=torch.float64)
+    want2 = torch.zeros((fine.total_voxels, C), dtype=torch.float64)
     rows, taps = np.nonzero(wmap >= 0)
     want2.index_add_(0, torch.from_numpy(wmap[rows, taps].astype(np.int64)), gs[torch.from_numpy(rows)])
     assert torch.allclose(xg2.grad.cpu().double(), want2.to(dtype).double(), rtol=tol, atol=tol)
@@ -95,11 +98,16 @@ def test_pool_dense_box_matches_torch():
 
 
 @pytest.mark.parametrize("f", [(2, 2, 2), (3, 1, 2)])
-def test_subdivide_and_inject_vs_oracle(f):
+@pytest.mark.parametrize("dtype,C", [(torch.float64, 4), (torch.float32, 3), (torch.float16, 3),
+                                     (torch.bfloat16, 8)])
+def test_subdivide_and_inject_vs_oracle(f, dtype, C):
+    """Rows copied in 16-byte units (f64 x 4, bf16 x 8), 4-byte units (f32 x 3) and bytes
+    (f16 x 3)."""
     fvdb = _fvdb()
     rng = np.random.default_rng(3)
+    print(f"[case] subdivide/inject f={f} {dtype} C={C}", flush=True)
     coarse, coarse_np = grids(fvdb, rng, [200, 90], 16)
-    vals = torch.from_numpy(rng.standard_normal((coarse.total_voxels, 4))).cuda()
+    vals = torch.from_numpy(rng.standard_normal((coarse.total_voxels, C))).to("cuda", dtype)
     up, fine = coarse.subdivide(f, coarse.jagged_like(vals))
     offs = fine.joffsets.cpu().tolist()
     fine_np = [fine.ijk.jdata[offs[b]:offs[b + 1]].cpu().numpy() for b in range(2)]
@@ -110,14 +118,14 @@ def test_subdivide_and_inject_vs_oracle(f):
     other, other_np = grids(fvdb, rng, [500, 200], 40)
     up2, _ = coarse.subdivide(f, coarse.jagged_like(vals), fine_grid=other)
     pidx2 = torch.from_numpy(oracle.parent_index(other_np, coarse_np, f))
-    want = torch.zeros((other.total_voxels, 4), dtype=torch.float64)
+    want = torch.zeros((other.total_voxels, C), dtype=dtype)
     ok = pidx2 >= 0
     want[ok] = vals.cpu()[pidx2[ok]]
     assert torch.equal(up2.jdata.cpu(), want)
     # inject coarse values onto the foreign grid
     inj = other.inject_from(coarse, coarse.jagged_like(vals), default_value=-7.0)
     iidx = torch.from_numpy(oracle.inject_index(other_np, coarse_np))
-    want = torch.full((other.total_voxels, 4), -7.0, dtype=torch.float64)
+    want = torch.full((other.total_voxels, C), -7.0, dtype=dtype)
     ok = iidx >= 0
     want[ok] = vals.cpu()[iidx[ok]]
     assert torch.equal(inj.jdata.cpu(), want)
