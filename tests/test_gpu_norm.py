@@ -92,6 +92,36 @@ def test_group_norm_vs_reference(dtype, C, G, affine, relu):
         assert rel_err(gn.weight.grad, wr.grad) <= gtol and rel_err(gn.bias.grad, br.grad) <= gtol
 
 
+@pytest.mark.parametrize("dtype,C,G", [(torch.float32, 1032, 8), (torch.bfloat16, 2064, 16)])
+def test_group_norm_wide_rows(dtype, C, G):
+    """Rows of more 16-byte segments than one workgroup covers (258 here): the statistics
+    kernels loop over segment chunks and the element-wise kernels take a second grid
+    dimension; forward and backward against float64 per grid."""
+    fvdb = _fvdb()
+    grid = batch_grid(fvdb, [900, 40, 700], seed=G)
+    counts = grid.num_voxels.tolist()
+    g = torch.Generator().manual_seed(C)
+    x = (torch.randn(grid.total_voxels, C, generator=g, dtype=torch.float64) * 1.5 - 0.4).to(dtype)
+    gn = fvdb.nn.GroupNorm(G, C).cuda().to(dtype)
+    with torch.no_grad():
+        gn.weight.copy_(torch.randn(C, generator=g).to(dtype))
+        gn.bias.copy_(torch.randn(C, generator=g).to(dtype))
+    R = torch.randn(grid.total_voxels, C, generator=g, dtype=torch.float64).to(dtype)
+    print(f"[group norm wide] {dtype} C={C} G={G}", flush=True)
+    xd = x.cuda().requires_grad_(True)
+    y = gn(fvdb.nn.VDBTensor(grid, grid.jagged_like(xd))).data.jdata
+    (y * R.cuda()).sum().backward()
+    xr = x.double().requires_grad_(True)
+    wr = gn.weight.detach().double().cpu().requires_grad_(True)
+    br = gn.bias.detach().double().cpu().requires_grad_(True)
+    ref = reference_group_norm(xr, counts, G, wr, br, gn.eps, False)
+    (ref * R.double()).sum().backward()
+    tol = TOL[dtype]
+    gtol = tol if dtype == torch.float32 else 2 * tol
+    assert rel_err(y, ref) <= tol and rel_err(xd.grad, xr.grad) <= gtol
+    assert rel_err(gn.weight.grad, wr.grad) <= gtol and rel_err(gn.bias.grad, br.grad) <= gtol
+
+
 def test_group_norm_dense_grid_vs_torch():
     """``tests/unit/test_nn.py:153-188``: 20^3 dense grid, 64/128 channels, 8-64 groups."""
     fvdb = _fvdb()
