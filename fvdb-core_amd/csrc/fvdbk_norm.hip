@@ -289,12 +289,12 @@ __global__ __launch_bounds__(NT) void k_gn_stats_partial_vec(const T* __restrict
 // k < GN_RPT, segment s0 + s with s0 = blockIdx.y * GN_THREADS) and loads the next chunk
 // before it writes the current one.  A thread's rows only increase, so its grid index walks
 // forward along the offsets from the grid of its first row (jidx; all rows are grid 0 without
-// it), and the statistics of a grid are loaded once per thread and grid.
+// it), never past grid B - 1, and the statistics of a grid are loaded once per thread and grid.
 
 // Pass 3 (vector form): y = (x - mean) * rstd * gamma + beta, the same arithmetic as k_gn_apply.
 template <typename T>
 __global__ __launch_bounds__(GN_THREADS) void k_gn_apply_vec(const T* __restrict__ x, int64_t n, int C, int G,
-                                                             const int64_t* __restrict__ off,
+                                                             const int64_t* __restrict__ off, int B,
                                                              const int32_t* __restrict__ jidx,
                                                              const typename NormAcc<T>::type* __restrict__ mean,
                                                              const typename NormAcc<T>::type* __restrict__ rstd,
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(GN_THREADS) void k_gn_apply_vec(const T* __restrict
     GnSeg<T> cur[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) cur[k].v = xs[min(row(ch, k), n - 1) * S];
-    int b = jidx ? jidx[min(row(ch, 0), n - 1)] : 0;
+    int b = jidx ? min(max(jidx[min(row(ch, 0), n - 1)], 0), B - 1) : 0;
     int64_t bend = off[b + 1];
     A mu[E], rs[E];
     auto load_grid = [&]() {
@@ -345,9 +345,9 @@ __global__ __launch_bounds__(GN_THREADS) void k_gn_apply_vec(const T* __restrict
         for (int k = 0; k < P; ++k) {
             const int64_t r = row(ch, k);
             if (r >= n) break;
-            if (r >= bend) {
+            if (r >= bend && b + 1 < B) {
                 do bend = off[++b + 1];
-                while (r >= bend);
+                while (r >= bend && b + 1 < B);
                 load_grid();
             }
             GnSeg<T> o;
@@ -596,7 +596,7 @@ __global__ void k_gn_bwd_dx(const T* __restrict__ x, const T* __restrict__ dy, c
 template <typename T, bool HAS_Y>
 __global__ __launch_bounds__(GN_THREADS) void k_gn_bwd_dx_vec(const T* __restrict__ x, const T* __restrict__ dy,
                                                               const T* __restrict__ y, int64_t n, int C, int G,
-                                                              const int64_t* __restrict__ off,
+                                                              const int64_t* __restrict__ off, int B,
                                                               const int32_t* __restrict__ jidx,
                                                               const typename NormAcc<T>::type* __restrict__ mean,
                                                               const typename NormAcc<T>::type* __restrict__ rstd,
@@ -635,7 +635,7 @@ __global__ __launch_bounds__(GN_THREADS) void k_gn_bwd_dx_vec(const T* __restric
         }
     };
     load_rows(ch, cx, cd, cy);
-    int b = jidx ? jidx[min(row(ch, 0), n - 1)] : 0;
+    int b = jidx ? min(max(jidx[min(row(ch, 0), n - 1)], 0), B - 1) : 0;
     int64_t bend = off[b + 1];
     A mu[E], rs[E], t1[E], t2[E];
     auto load_grid = [&]() {
@@ -658,9 +658,9 @@ __global__ __launch_bounds__(GN_THREADS) void k_gn_bwd_dx_vec(const T* __restric
         for (int k = 0; k < P; ++k) {
             const int64_t r = row(ch, k);
             if (r >= n) break;
-            if (r >= bend) {
+            if (r >= bend && b + 1 < B) {
                 do bend = off[++b + 1];
-                while (r >= bend);
+                while (r >= bend && b + 1 < B);
                 load_grid();
             }
             GnSeg<T> o;
@@ -741,7 +741,7 @@ static int gn_forward(const T* x, int64_t n, int C, int G, const int64_t* off, c
     if (n > 0) {
         if (vec)
             hipLaunchKernelGGL((k_gn_apply_vec<T>), gn_vec_grid<T>(k_gn_apply_vec<T>, n, C), dim3(GN_THREADS), 0, s,
-                               x, n, C, G, off, jidx, (const A*)mean, (const A*)rstd, gamma, beta, relu, y);
+                               x, n, C, G, off, B, jidx, (const A*)mean, (const A*)rstd, gamma, beta, relu, y);
         else
             hipLaunchKernelGGL((k_gn_apply<T>), dim3(grid_1d(n * C, 256)), dim3(256), 0, s, x, n, C, G, jidx,
                                (const A*)mean, (const A*)rstd, gamma, beta, relu, y);
@@ -791,12 +791,12 @@ static int gn_backward(const T* x, const T* dy, const T* y, int64_t n, int C, in
     if (n > 0) {
         if (vec && y)
             hipLaunchKernelGGL((k_gn_bwd_dx_vec<T, true>), gn_vec_grid<T>(k_gn_bwd_dx_vec<T, true>, n, C),
-                               dim3(GN_THREADS), 0, s, x, dy, y, n, C, G, off, jidx, (const A*)mean, (const A*)rstd,
-                               gamma, gsums, dx);
+                               dim3(GN_THREADS), 0, s, x, dy, y, n, C, G, off, B, jidx, (const A*)mean,
+                               (const A*)rstd, gamma, gsums, dx);
         else if (vec)
             hipLaunchKernelGGL((k_gn_bwd_dx_vec<T, false>), gn_vec_grid<T>(k_gn_bwd_dx_vec<T, false>, n, C),
-                               dim3(GN_THREADS), 0, s, x, dy, y, n, C, G, off, jidx, (const A*)mean, (const A*)rstd,
-                               gamma, gsums, dx);
+                               dim3(GN_THREADS), 0, s, x, dy, y, n, C, G, off, B, jidx, (const A*)mean,
+                               (const A*)rstd, gamma, gsums, dx);
         else
             hipLaunchKernelGGL((k_gn_bwd_dx<T>), dim3(grid_1d(n * C, 256)), dim3(256), 0, s, x, dy, y, n, C, G, off,
                                jidx, (const A*)mean, (const A*)rstd, gamma, gsums, dx);
