@@ -122,6 +122,39 @@ def test_group_norm_wide_rows(dtype, C, G):
     assert rel_err(gn.weight.grad, wr.grad) <= gtol and rel_err(gn.bias.grad, br.grad) <= gtol
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_group_norm_empty_grids_in_batch(dtype):
+    """Grids without voxels inside and at the end of a batch: the per-thread grid walk steps
+    over them and the other grids' results equal their results alone."""
+    fvdb = _fvdb()
+    g = torch.Generator().manual_seed(5)
+    parts = []
+    for n in (300, 0, 500, 0):
+        ijk = torch.unique(torch.randint(0, 12, (n * 3, 3), generator=g, dtype=torch.int32), dim=0)[:n]
+        parts.append(ijk.reshape(-1, 3).cuda())
+    grid = fvdb.GridBatch.from_ijk(fvdb.JaggedTensor(parts))
+    counts = grid.num_voxels.tolist()
+    assert counts[1] == 0 and counts[3] == 0
+    C, G = 32, 4
+    x = (torch.randn(grid.total_voxels, C, generator=g, dtype=torch.float64) + 0.3).to(dtype)
+    R = torch.randn(grid.total_voxels, C, generator=g, dtype=torch.float64).to(dtype)
+    gn = fvdb.nn.GroupNorm(G, C).cuda().to(dtype)
+    xd = x.cuda().requires_grad_(True)
+    y = gn(fvdb.nn.VDBTensor(grid, grid.jagged_like(xd))).data.jdata
+    (y * R.cuda()).sum().backward()
+    xr = x.double().requires_grad_(True)
+    ref = reference_group_norm(xr, counts, G, gn.weight.detach().double().cpu(), gn.bias.detach().double().cpu(),
+                               gn.eps, False)
+    (ref * R.double()).sum().backward()
+    tol = TOL[dtype]
+    assert rel_err(y, ref) <= tol and rel_err(xd.grad, xr.grad) <= 2 * tol
+    jo = grid.joffsets.cpu().tolist()
+    for b in (0, 2):
+        single = fvdb.GridBatch.from_ijk(grid.ijk.jdata[jo[b]:jo[b + 1]])
+        ys = gn(fvdb.nn.VDBTensor(single, single.jagged_like(xd.detach()[jo[b]:jo[b + 1]]))).data.jdata
+        assert torch.equal(ys, y.detach()[jo[b]:jo[b + 1]])
+
+
 def test_group_norm_dense_grid_vs_torch():
     """``tests/unit/test_nn.py:153-188``: 20^3 dense grid, 64/128 channels, 8-64 groups."""
     fvdb = _fvdb()
