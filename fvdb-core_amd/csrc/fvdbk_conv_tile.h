@@ -1447,8 +1447,8 @@ static void launch_tile(const T* X, const T* A, const int32_t* map, const int32_
 constexpr size_t TILE_LDS_LIMIT = 65536;
 
 // Channel tile for a compile-time CIN: 64 output channels x 128 rows when the LDS
-// fits (the best shape measured on MI355X), else 32 x 128 rows for fp32 and 32 x 256 or
-// 16 x 256 rows otherwise.
+// fits (the best shape measured on MI355X; 64 rows for fp32), else 32 x 128 rows for
+// fp32 and 32 x 256 or 16 x 256 rows otherwise.
 template <typename T, int CIN>
 static bool try_tile(const T* X, int64_t n_in, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows,
                      int K, int M, T* Y, const T* bias, int relu, hipStream_t s) {
@@ -1462,7 +1462,12 @@ static bool try_tile(const T* X, int64_t n_in, const T* A, const int32_t* map, c
         // (2x2x2 stride-2 64->128 of the sphere batch: 0.19 -> 0.16 ms on MI355X)
         if (perm == nullptr && launch_tile_dma<T>(X, n_in, A, map, n_rows, K, M, Y, bias, relu, s)) return true;
     }
-    if (M > 32 && tile_lds_bytes<T, 4, 2, CIN>(K) <= TILE_LDS_LIMIT) {
+    if (sizeof(T) == 4 && M > 32 && tile_lds_bytes<T, 4, 1, CIN>(K) <= TILE_LDS_LIMIT) {
+        // fp32 with more than 32 outputs in 64-row tiles (2.3M-voxel sphere batch, 3x3x3
+        // 64->64: 4.10 -> 3.72 ms through the windowed row order, 4.32 -> 4.23 ms in tree
+        // order; the fp32 U-Net 19.6 -> 19.2 ms)
+        launch_tile<T, 4, 1, CIN>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
+    } else if (M > 32 && tile_lds_bytes<T, 4, 2, CIN>(K) <= TILE_LDS_LIMIT) {
         launch_tile<T, 4, 2, CIN>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     } else if (sizeof(T) == 4 && M > 16 && tile_lds_bytes<T, 2, 2, CIN>(K) <= TILE_LDS_LIMIT) {
         // fp32 with 17-32 outputs in 128-row tiles: twice the workgroups of the 256-row shape
