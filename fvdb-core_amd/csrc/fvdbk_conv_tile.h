@@ -1556,7 +1556,10 @@ static bool try_tile_c(const T* X, const T* A, const int32_t* map, const int32_t
             return true;
         }
     }
-    if (M > 32) launch_tile_c<T, 4, 2, NCC>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
+    // fp32 with more than 32 outputs in 64-row tiles (1.1M-voxel 3x3x3 128->128 layer through
+    // the windowed row order: forward and dgrad 5.88 -> 5.31 ms)
+    if (sizeof(T) == 4 && M > 32) launch_tile_c<T, 4, 1, NCC>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
+    else if (M > 32) launch_tile_c<T, 4, 2, NCC>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     else if (M > 16) launch_tile_c<T, 2, 4, NCC>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     else launch_tile_c<T, 1, 4, NCC>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     return true;
