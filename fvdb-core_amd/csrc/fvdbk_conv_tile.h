@@ -1447,7 +1447,7 @@ static void launch_tile(const T* X, const T* A, const int32_t* map, const int32_
 constexpr size_t TILE_LDS_LIMIT = 65536;
 
 // Channel tile for a compile-time CIN: 64 output channels x 128 rows when the LDS
-// fits (the best shape measured on MI355X; 64 rows for fp32), else 32 x 128 rows for
+// fits (the best shape measured on MI355X; 64 rows for fp32), else 32 x 64 rows for
 // fp32 and 32 x 256 or 16 x 256 rows otherwise.
 template <typename T, int CIN>
 static bool try_tile(const T* X, int64_t n_in, const T* A, const int32_t* map, const int32_t* perm, int64_t n_rows,
@@ -1469,10 +1469,11 @@ static bool try_tile(const T* X, int64_t n_in, const T* A, const int32_t* map, c
         launch_tile<T, 4, 1, CIN>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     } else if (M > 32 && tile_lds_bytes<T, 4, 2, CIN>(K) <= TILE_LDS_LIMIT) {
         launch_tile<T, 4, 2, CIN>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
-    } else if (sizeof(T) == 4 && M > 16 && tile_lds_bytes<T, 2, 2, CIN>(K) <= TILE_LDS_LIMIT) {
-        // fp32 with 17-32 outputs in 128-row tiles: twice the workgroups of the 256-row shape
-        // (one 338k-voxel grid, 3x3x3 32->32: 0.255 -> 0.225 ms; the fp32 U-Net 20.6 -> 19.5 ms)
-        launch_tile<T, 2, 2, CIN>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
+    } else if (sizeof(T) == 4 && M > 16 && tile_lds_bytes<T, 2, 1, CIN>(K) <= TILE_LDS_LIMIT) {
+        // fp32 with 17-32 outputs in 64-row tiles (one 338k-voxel grid, 3x3x3 32->32: 0.255 ms
+        // in 256-row tiles, 0.225 in 128-row, 0.203 in 64-row; the fp32 U-Net 18.1 -> 17.5 ms
+        // against 128-row tiles)
+        launch_tile<T, 2, 1, CIN>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     } else if (M > 16 && tile_lds_bytes<T, 2, 4, CIN>(K) <= TILE_LDS_LIMIT) {
         launch_tile<T, 2, 4, CIN>(X, A, map, perm, n_rows, K, M, Y, bias, relu, s);
     } else if (tile_lds_bytes<T, 1, 4, CIN>(K) <= TILE_LDS_LIMIT) {
