@@ -738,6 +738,117 @@ static int kmap_box_bound(const ProbeGeom& g) {
     return nb;
 }
 
+// Maps of 2x2x2 kernels with stride 2 (K = 8, tap k = 4 dx + 2 dy + dz, dz fastest), one
+// wavefront per row leaf as in k_kmap, with the geometry fixed so each row is a few LDS reads:
+//   DOWN (rows: coarse voxels c, probe: fine grid): tap (dx, dy, dz) -> fine voxel 2c + d.  The
+//        row leaf's children fill the 2x2x2 fine leaves at 2 * origin + 8 * (0|1), staged in LDS;
+//        a row's 8 children lie in one of them, the four (dy, dz) of each dx in one 32-bit half
+//        of a mask word.
+//   UP (rows: fine voxels v, probe: coarse grid; the transposed map): the one valid tap of v is
+//        its position d = v & 1 inside its parent v >> 1, and every parent of the row leaf lies
+//        in the single coarse leaf at (origin >> 1) rounded down to a multiple of 8.
+// Entries are bit-identical to k_kmap's.  Each row's 8 entries are written as two 16-byte stores.
+// A workgroup takes KMAP_S2_WAVES row leaves, one per wavefront (U-Net topology of 3.2M voxels:
+// 123 us per forward for all six 2x2x2 maps with one leaf per workgroup, 129 us with four;
+// k_kmap took 253 us for them).
+constexpr int KMAP_S2_WAVES = 1;  // row leaves per workgroup of k_kmap_s2, one per wavefront
+
+template <bool UP>
+__global__ __launch_bounds__(64 * KMAP_S2_WAVES) void k_kmap_s2(fvdbk_grid rows, fvdbk_grid probe, int n_groups,
+                                                                int tail_len, int32_t* __restrict__ map) {
+    __shared__ uint32_t s_vox_w[KMAP_S2_WAVES][512];
+    __shared__ uint2 s_mb_w[KMAP_S2_WAVES][8 * 16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t grp = xcd_swizzle(blockIdx.x, n_groups);
+    if (grp < 0) {
+        // the launch's last workgroup (always a padding one when tail_len > 0) sets the map
+        // entries of the padding rows past the last voxel to -1
+        if (tail_len > 0 && blockIdx.x == gridDim.x - 1)
+            for (int i = threadIdx.x; i < tail_len; i += 64 * KMAP_S2_WAVES) map[rows.n_voxels * 8 + i] = -1;
+        return;
+    }
+    const int64_t L = grp * KMAP_S2_WAVES + __builtin_amdgcn_readfirstlane(wave);
+    const bool live = L < rows.n_leaves;
+    uint32_t* s_vox = s_vox_w[wave];
+    uint2* s_mb = s_mb_w[wave];
+    int32_t b = 0, o[3] = {0, 0, 0};
+    int64_t vbase = 0;
+    int cnt = 0;
+    if (live) {
+        decode_leaf_key(rows.leaf_keys[L], b, o[0], o[1], o[2]);
+        vbase = rows.leaf_word_base[L * 8];
+        int rank = 0;
+        for (int w = 0; w < 8; ++w) {
+            const uint64_t m = rows.leaf_masks[L * 8 + w];
+            if ((m >> lane) & 1ull)
+                s_vox[rank + __popcll(m & ((1ull << lane) - 1ull))] =
+                    (uint32_t)w | ((uint32_t)(lane >> 3) << 8) | ((uint32_t)(lane & 7) << 16);
+            rank += __popcll(m);
+        }
+        cnt = rank;
+        // stage the probe leaves: 8 fine leaves (DOWN) or the one coarse leaf (UP)
+        constexpr int NS = UP ? 1 : 8;
+        if (lane < NS) {
+            int32_t q[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int d = UP ? 0 : (lane >> (2 - a)) & 1;
+                q[a] = UP ? ((o[a] >> 1) & ~7) : 2 * o[a] + 8 * d;
+            }
+            int32_t leaf = -1;
+            if (probe.n_leaves > 0 && q[0] >= kCoordMin && q[0] <= kCoordMax && q[1] >= kCoordMin &&
+                q[1] <= kCoordMax && q[2] >= kCoordMin && q[2] <= kCoordMax)
+                leaf = hash_find(probe.hash, probe.hash_cap, leaf_key(b, q[0], q[1], q[2]));
+#pragma unroll
+            for (int w = 0; w < 8; ++w) {
+                const uint64_t m = leaf >= 0 ? probe.leaf_masks[(int64_t)leaf * 8 + w] : 0ull;
+                const uint32_t bw = leaf >= 0 ? (uint32_t)probe.leaf_word_base[(int64_t)leaf * 8 + w] : 0u;
+                const uint32_t lo32 = (uint32_t)m, hi32 = (uint32_t)(m >> 32);
+                s_mb[lane * 16 + 2 * w] = make_uint2(lo32, bw);
+                s_mb[lane * 16 + 2 * w + 1] = make_uint2(hi32, bw + __popc(lo32));
+            }
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    int4* out = reinterpret_cast<int4*>(map + vbase * 8);
+    // index of bit `sh` of a staged 32-bit half-word, or -1
+    auto idx = [](uint2 hw, int sh) -> int32_t {
+        return ((hw.x >> sh) & 1u) ? (int32_t)(hw.y + __popc(hw.x & ((1u << sh) - 1u))) : -1;
+    };
+    for (int r = lane; r < cnt; r += 64) {
+        const uint32_t v = s_vox[r];
+        const int lx = (int)(v & 255u), ly = (int)((v >> 8) & 255u), lz = (int)(v >> 16);
+        int32_t e[8];
+        if constexpr (!UP) {
+            // children 2 (o + l) + d: fine leaf (l >> 2) of the box, local 2 (l & 3) + d
+            const int n = ((lx >> 2) << 2) | ((ly >> 2) << 1) | (lz >> 2);
+            const int fx = 2 * (lx & 3), fy = 2 * (ly & 3), fz = 2 * (lz & 3);
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                // bits (fy + dy) * 8 + fz + dz of word fx + dx, all in its half fy >> 2
+                const uint2 hw = s_mb[n * 16 + 2 * (fx + dx) + (fy >> 2)];
+                const int sh0 = ((fy & 3) << 3) + fz;
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                    for (int dz = 0; dz < 2; ++dz) e[dx * 4 + dy * 2 + dz] = idx(hw, sh0 + dy * 8 + dz);
+            }
+        } else {
+            // parent local coordinates in the staged coarse leaf
+            const int px = ((o[0] >> 1) & 7) + (lx >> 1), py = ((o[1] >> 1) & 7) + (ly >> 1),
+                      pz = ((o[2] >> 1) & 7) + (lz >> 1);
+            const int bit = px * 64 + py * 8 + pz;
+            const int32_t p = idx(s_mb[bit >> 5], bit & 31);
+            const int k = (lx & 1) * 4 + (ly & 1) * 2 + (lz & 1);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) e[t] = t == k ? p : -1;
+        }
+        out[2 * r] = make_int4(e[0], e[1], e[2], e[3]);
+        out[2 * r + 1] = make_int4(e[4], e[5], e[6], e[7]);
+    }
+}
+
 __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1164,6 +1275,34 @@ static int kmap_launch(const fvdbk_grid* rows, const fvdbk_grid* probe, ProbeGeo
         g.shift[a] = -1;
         for (int sh = 0; sh < 31; ++sh)
             if ((1 << sh) == g.div[a]) g.shift[a] = sh;
+    }
+    // 2x2x2 kernels with stride 2, both directions: the fixed-geometry kernel
+    bool s2_down = g.K == 8 && g.sgn == 1, s2_up = g.K == 8 && g.sgn == -1;
+    for (int a = 0; a < 3; ++a) {
+        s2_down = s2_down && g.ext[a] == 2 && g.off0[a] == 0 && g.mul[a] == 2 && g.div[a] == 1;
+        s2_up = s2_up && g.ext[a] == 2 && g.off0[a] == 0 && g.mul[a] == 1 && g.div[a] == 2;
+    }
+    if ((s2_down || s2_up) && ((uintptr_t)map % 16) == 0) {
+        const int64_t tail = (n_rows_padded - rows->n_voxels) * K;
+        const bool fold_tail = rows->n_leaves > 0 && tail > 0 && tail <= 8192;
+        if (rows->n_leaves > 0) {
+            const int ng = (int)((rows->n_leaves + KMAP_S2_WAVES - 1) / KMAP_S2_WAVES);
+            const unsigned blocks = (unsigned)((ng + (fold_tail ? 1 : 0) + 7) / 8 * 8);
+            const int tl = fold_tail ? (int)tail : 0;
+            if (s2_down)
+                hipLaunchKernelGGL(k_kmap_s2<false>, dim3(blocks), dim3(64 * KMAP_S2_WAVES), 0, stream, *rows, *probe,
+                                   ng, tl, map);
+            else
+                hipLaunchKernelGGL(k_kmap_s2<true>, dim3(blocks), dim3(64 * KMAP_S2_WAVES), 0, stream, *rows, *probe,
+                                   ng, tl, map);
+            FVDBK_LAUNCH_CHECK("k_kmap_s2");
+        }
+        if (tail > 0 && !fold_tail) {
+            hipLaunchKernelGGL(k_fill_i32, dim3(grid_1d(tail, 256)), dim3(256), 0, stream, map + rows->n_voxels * K,
+                               tail, -1);
+            FVDBK_LAUNCH_CHECK("k_fill_i32");
+        }
+        return FVDBK_OK;
     }
     if (rows->n_leaves > 0) {
         int nb = kmap_box_bound(g);
