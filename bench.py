@@ -685,9 +685,10 @@ def run_orders(torch, fvdb, dev, args):
         g = torch.Generator(device=dev).manual_seed(3)
         x64 = torch.randn((n_in, cin), generator=g, device=dev, dtype=torch.float64)
         a64 = torch.randn((K, cout, cin), generator=g, device=dev, dtype=torch.float64) / math.sqrt(cin * K)
-        order = ext.row_order(m, n_rows, AUTO_ORDER_WINDOW)
-        rec = {"layer": name, "rows": n_rows, "taps": K, "order_ms": _median_ms(torch, lambda: ext.row_order(
-            m, n_rows, AUTO_ORDER_WINDOW), 10)}
+        win = AUTO_ORDER_WINDOW
+        order = ext.row_order(m, n_rows, win)
+        rec = {"layer": name, "rows": n_rows, "taps": K, "window": win,
+               "order_ms": _median_ms(torch, lambda: ext.row_order(m, n_rows, win), 10)}
         for dn, dt in (("bf16", torch.bfloat16), ("fp32", torch.float32)):
             x, a = x64.to(dt), a64.to(dt)
             ref = ext.implicit_gemm(x, a, m, n_rows)

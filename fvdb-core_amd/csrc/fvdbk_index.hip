@@ -909,25 +909,27 @@ __global__ __launch_bounds__(NB_THREADS) void k_nbmap_emit(const int32_t* __rest
 
 // ---------------------------------------------------------------------------
 // Row order by tap mask within windows of consecutive rows.  One workgroup owns
-// ORDER_BLOCK consecutive rows: it builds each row's key (window index within the
+// ORDER_THREADS * ITEMS consecutive rows (8192, or 2048 for windows of up to 2048 rows, so
+// that small grids still spread over many workgroups): it builds each row's key (window index within the
 // block above the inverted tap mask of the first min(K, 27) taps, so windows never mix
 // and masks sort descending), runs a block-wide radix sort over just those key bits
 // (stable: ties keep row order), and writes the rows in sorted order.  Rows past
 // n_rows take the largest key and, being last in the block, stay behind every real row.
 // ---------------------------------------------------------------------------
 constexpr int ORDER_THREADS = 1024;
-constexpr int ORDER_ITEMS = 8;
-constexpr int ORDER_BLOCK = ORDER_THREADS * ORDER_ITEMS;
-constexpr int ORDER_LOG2_BLOCK = 13;
+constexpr int ORDER_BLOCK = ORDER_THREADS * 8;  // largest block (and window)
+constexpr int ORDER_SMALL_BLOCK = ORDER_THREADS * 2;
 constexpr int ORDER_MASK_BITS = 27;
-static_assert(ORDER_BLOCK == 1 << ORDER_LOG2_BLOCK, "block size");
 
 // KT > 0: compile-time kernel volume (the tap loads of a row are unrolled and issued
-// together); KT = 0: runtime K.
-template <int KT>
+// together); KT = 0: runtime K.  ORDER_ITEMS rows per lane.
+template <int KT, int ORDER_ITEMS>
 __global__ __launch_bounds__(ORDER_THREADS) void k_row_order_window(const int32_t* __restrict__ map, int64_t n_rows,
                                                                     int K_rt, int log2_window,
                                                                     int32_t* __restrict__ order) {
+    constexpr int ORDER_BLOCK = ORDER_THREADS * ORDER_ITEMS;
+    constexpr int ORDER_LOG2_BLOCK = ORDER_ITEMS == 8 ? 13 : (ORDER_ITEMS == 2 ? 11 : -1);
+    static_assert(ORDER_LOG2_BLOCK > 0 && ORDER_BLOCK == 1 << ORDER_LOG2_BLOCK, "block size");
     using Sort = hipcub::BlockRadixSort<uint32_t, ORDER_THREADS, ORDER_ITEMS, int32_t>;
     __shared__ union {
         typename Sort::TempStorage sort;
@@ -1374,14 +1376,26 @@ int fvdbk_kmap_row_order_windowed(const int32_t* map, int64_t n_rows, int32_t K,
     FVDBK_REQUIRE(n_rows >= 0 && n_rows < ((int64_t)1 << 31), FVDBK_EINVAL, "row count out of range");
     if (n_rows == 0) return FVDBK_OK;
     const int log2w = __builtin_ctz((unsigned)window);
-    const int64_t blocks = (n_rows + ORDER_BLOCK - 1) / ORDER_BLOCK;
+    // windows of up to 2048 rows: 2048-row workgroups (4x the workgroups of 8192-row ones)
+    const bool small = window <= ORDER_SMALL_BLOCK;
+    const int64_t rows_per_block = small ? ORDER_SMALL_BLOCK : ORDER_BLOCK;
+    const int64_t blocks = (n_rows + rows_per_block - 1) / rows_per_block;
     const dim3 grid((unsigned)blocks), block(ORDER_THREADS);
-    if (K == 27)
-        hipLaunchKernelGGL(k_row_order_window<27>, grid, block, 0, stream, map, n_rows, (int)K, log2w, order);
-    else if (K == 8)
-        hipLaunchKernelGGL(k_row_order_window<8>, grid, block, 0, stream, map, n_rows, (int)K, log2w, order);
-    else
-        hipLaunchKernelGGL(k_row_order_window<0>, grid, block, 0, stream, map, n_rows, (int)K, log2w, order);
+    if (small) {
+        if (K == 27)
+            hipLaunchKernelGGL((k_row_order_window<27, 2>), grid, block, 0, stream, map, n_rows, (int)K, log2w, order);
+        else if (K == 8)
+            hipLaunchKernelGGL((k_row_order_window<8, 2>), grid, block, 0, stream, map, n_rows, (int)K, log2w, order);
+        else
+            hipLaunchKernelGGL((k_row_order_window<0, 2>), grid, block, 0, stream, map, n_rows, (int)K, log2w, order);
+    } else {
+        if (K == 27)
+            hipLaunchKernelGGL((k_row_order_window<27, 8>), grid, block, 0, stream, map, n_rows, (int)K, log2w, order);
+        else if (K == 8)
+            hipLaunchKernelGGL((k_row_order_window<8, 8>), grid, block, 0, stream, map, n_rows, (int)K, log2w, order);
+        else
+            hipLaunchKernelGGL((k_row_order_window<0, 8>), grid, block, 0, stream, map, n_rows, (int)K, log2w, order);
+    }
     FVDBK_LAUNCH_CHECK("k_row_order_window");
     return FVDBK_OK;
 }

@@ -681,7 +681,12 @@ def test_automatic_row_order_policy():
     assert wants_row_order(27, 128, f32, 128, 300_000) and wants_row_order(8, 128, bf, 128, 300_000)
     o = p2._order_for(False, 64, f32)                  # fine rows in fp32: windowed order
     assert o is not None
-    assert np.array_equal(o.cpu().numpy(), oracle.row_order(p2._map_bwd()[:grid.total_voxels].cpu().numpy(), 8192))
+    mb = p2._map_bwd()[:grid.total_voxels].cpu().numpy()
+    assert np.array_equal(o.cpu().numpy(), oracle.row_order(mb, 8192))
+    # both workgroup sizes of the order kernel (2048- and 8192-row blocks) against the oracle
+    for w in (256, 2048, 4096, 8192):
+        got = fvdb.ext.row_order(p2._map_bwd(), grid.total_voxels, w)
+        assert np.array_equal(got.cpu().numpy(), oracle.row_order(mb, w)), w
     g = torch.Generator().manual_seed(8)
     for dtype in (bf, f32):
         for pack, n_in, transposed, cin, cout in ((p3, grid.total_voxels, False, 128, 64),
